@@ -1,0 +1,313 @@
+"""Python mirror of the C ABI of libNativeSPIMReconstructionCuda.so (gfx950).
+
+The product is the shared library built from ``csrc/``; this module only
+loads it with ctypes and mirrors the reference interface
+(bene51/gpu_deconvolution ``fmvd_deconvolve.impl.h:6-66``,
+``fmvd_transform.h:4-24``, ``fmvd_cuda_utils.h:4-25``) so tests and the
+benchmark call exactly what a C or JNI caller calls.  There is no CPU
+fallback: importing ``lib()`` raises if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libNativeSPIMReconstructionCuda.so")
+
+PSF_TYPES = {"independent": 0, "efficient_bayesian": 1, "optimization_1": 2, "optimization_2": 3}
+
+ERROR_HANDLER_T = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
+SOURCE16_T = C.CFUNCTYPE(C.c_int, C.POINTER(C.POINTER(C.c_uint16)), C.c_int)
+SINK16_T = C.CFUNCTYPE(None, C.POINTER(C.c_uint16))
+SOURCE8_T = C.CFUNCTYPE(C.c_int, C.POINTER(C.POINTER(C.c_uint8)), C.c_int)
+SINK8_T = C.CFUNCTYPE(None, C.POINTER(C.c_uint8))
+ESINK_T = C.CFUNCTYPE(None, C.POINTER(C.c_float), C.c_int, C.c_void_p)
+
+TIMING_KINDS = {"column": 0, "ratio": 1, "update": 2, "ingest": 3, "egress": 4, "transform": 5}
+
+_lib = None
+_errors: list = []
+
+
+class FmvdError(RuntimeError):
+    pass
+
+
+def _record_error(_param, msg):
+    _errors.append(msg.decode() if msg else "")
+
+
+_error_cb = ERROR_HANDLER_T(_record_error)
+
+
+def lib() -> C.CDLL:
+    """Load the library (raises if it was not built) and install a
+    non-exiting error handler that records messages for :func:`check`."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                          "(make -C gpu_deconvolution_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, f, cp = C.c_void_p, C.c_int, C.c_float, C.c_char_p
+    fpp = C.POINTER(C.POINTER(C.c_float))
+    sigs = {
+        "setErrorHandler": (None, [ERROR_HANDLER_T, vp]),
+        "iDivUp": (i, [i, i]),
+        "getNumCUDADevices": (i, []),
+        "getCudaDeviceName": (None, [i, cp]),
+        "setCudaDevice": (None, [i]),
+        "fmvd_last_error": (i, [cp, i]),
+        "fmvd_malloc": (vp, [C.c_size_t]),
+        "fmvd_free": (None, [vp]),
+        "fmvd_initialize_cuda_16": (vp, [i, i, fpp, fpp, i, i, i, i, i, SOURCE16_T, SINK16_T]),
+        "fmvd_initialize_cuda_8": (vp, [i, i, fpp, fpp, i, i, i, i, i, SOURCE8_T, SINK8_T]),
+        "fmvd_deconvolve_planes_cuda_16": (None, [vp, i]),
+        "fmvd_deconvolve_planes_cuda_8": (None, [vp, i]),
+        "fmvd_destroy_cuda_16": (None, [vp]),
+        "fmvd_destroy_cuda_8": (None, [vp]),
+        "fmvd_deconvolve_files_cuda_16": (None, [vp, vp, i, i, i, fpp, fpp, i, i, i, i, i]),
+        "fmvd_deconvolve_files_cuda_8": (None, [vp, vp, i, i, i, fpp, fpp, i, i, i, i, i]),
+        "fmvd_set_estimate_sink_cuda_16": (i, [vp, ESINK_T, vp]),
+        "fmvd_set_estimate_sink_cuda_8": (i, [vp, ESINK_T, vp]),
+        "fmvd_set_batch_planes_cuda_16": (i, [vp, i]),
+        "fmvd_set_batch_planes_cuda_8": (i, [vp, i]),
+        "fmvd_deconvolve_volume_device_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
+        "fmvd_deconvolve_volume_device_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
+        "fmvd_plan_info_cuda_16": (i, [vp, C.POINTER(C.c_int)]),
+        "fmvd_plan_info_cuda_8": (i, [vp, C.POINTER(C.c_int)]),
+        "fmvd_snap_transform_size": (i, [i]),
+        "fmvd_fft_extent_supported": (i, [i, i]),
+        "fmvd_kernel_timing_enable": (None, [i]),
+        "fmvd_kernel_timing_reset": (None, []),
+        "fmvd_kernel_timing_read": (i, [i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+        "transform_cuda_16": (None, [vp, i, i, i, i, i, i, C.POINTER(f), cp, i, i, f, cp]),
+        "transform_cuda_8": (None, [vp, i, i, i, i, i, i, C.POINTER(f), cp, i, i, f, cp]),
+        "fmvd_transform_device_16": (i, [vp, i, i, i, vp, i, i, i, C.POINTER(f), vp]),
+        "fmvd_transform_device_8": (i, [vp, i, i, i, vp, i, i, i, C.POINTER(f), vp]),
+        "fmvd_transform_mask_device": (i, [i, i, i, i, i, i, C.POINTER(f), i, f, vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    L.setErrorHandler(_error_cb, None)
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list:
+    return ["setErrorHandler", "iDivUp", "getNumCUDADevices", "getCudaDeviceName", "setCudaDevice",
+            "fmvd_malloc", "fmvd_free", "fmvd_initialize_cuda_16", "fmvd_initialize_cuda_8",
+            "fmvd_deconvolve_planes_cuda_16", "fmvd_deconvolve_planes_cuda_8",
+            "fmvd_destroy_cuda_16", "fmvd_destroy_cuda_8", "fmvd_deconvolve_files_cuda_16",
+            "fmvd_deconvolve_files_cuda_8", "transform_cuda_16", "transform_cuda_8"]
+
+
+def take_errors() -> list:
+    out = list(_errors)
+    _errors.clear()
+    return out
+
+
+def check():
+    """Raise FmvdError if the library reported an error since the last call."""
+    errs = take_errors()
+    if errs:
+        raise FmvdError("; ".join(errs))
+
+
+def snap_transform_size(n: int) -> int:
+    return lib().fmvd_snap_transform_size(int(n))
+
+
+def fft_extent_supported(ffth: int, fftw: int) -> bool:
+    return bool(lib().fmvd_fft_extent_supported(int(ffth), int(fftw)))
+
+
+def _float_ptr_array(arrays: Sequence[np.ndarray]):
+    arr_t = C.POINTER(C.c_float) * len(arrays)
+    return arr_t(*[a.ctypes.data_as(C.POINTER(C.c_float)) for a in arrays])
+
+
+class Deconvolver:
+    """Mirror of the plan API (fmvd_initialize / fmvd_deconvolve_planes /
+    fmvd_destroy) with in-memory plane streaming.
+
+    kernels: V arrays (kh, kw) float32 -- normalised IN PLACE, as the C API does.
+    weights: V arrays (dh, dw) float32.
+    """
+
+    def __init__(self, data_h: int, data_w: int, weights: Sequence[np.ndarray],
+                 kernels: Sequence[np.ndarray], psf_type: int = 0, bits: int = 16, nstreams: int = 3,
+                 batch_planes: int = 0):
+        L = lib()
+        self.bits = bits
+        self.h, self.w = int(data_h), int(data_w)
+        self.nviews = len(kernels)
+        self.kernels = [np.ascontiguousarray(k, dtype=np.float32) for k in kernels]
+        self.weights = [np.ascontiguousarray(w, dtype=np.float32) for w in weights]
+        kh, kw = self.kernels[0].shape
+        self._dtype = np.uint16 if bits == 16 else np.uint8
+        self._ctype = C.c_uint16 if bits == 16 else C.c_uint8
+        self._planes_in: list = []
+        self._planes_out: list = []
+        self._est_out: list = []
+        self._pos = 0
+        plane = self.h * self.w
+
+        def source(bufs, offset):
+            if self._pos >= len(self._planes_in):
+                return 0
+            views = self._planes_in[self._pos]
+            for v in range(self.nviews):
+                dst = np.ctypeslib.as_array(C.cast(bufs[v], C.POINTER(self._ctype)), shape=(offset + plane,))
+                dst[offset:offset + plane] = views[v].reshape(-1)
+            self._pos += 1
+            return 1
+
+        def sink(buf):
+            self._planes_out.append(np.ctypeslib.as_array(buf, shape=(plane,)).copy().reshape(self.h, self.w))
+
+        def esink(buf, z, _user):
+            self._est_out.append(np.ctypeslib.as_array(buf, shape=(plane,)).copy().reshape(self.h, self.w))
+
+        if bits == 16:
+            self._src, self._snk = SOURCE16_T(source), SINK16_T(sink)
+            init, self._run, self._destroy = L.fmvd_initialize_cuda_16, L.fmvd_deconvolve_planes_cuda_16, L.fmvd_destroy_cuda_16
+            self._set_esink, self._set_batch = L.fmvd_set_estimate_sink_cuda_16, L.fmvd_set_batch_planes_cuda_16
+            self._volume, self._info = L.fmvd_deconvolve_volume_device_cuda_16, L.fmvd_plan_info_cuda_16
+        elif bits == 8:
+            self._src, self._snk = SOURCE8_T(source), SINK8_T(sink)
+            init, self._run, self._destroy = L.fmvd_initialize_cuda_8, L.fmvd_deconvolve_planes_cuda_8, L.fmvd_destroy_cuda_8
+            self._set_esink, self._set_batch = L.fmvd_set_estimate_sink_cuda_8, L.fmvd_set_batch_planes_cuda_8
+            self._volume, self._info = L.fmvd_deconvolve_volume_device_cuda_8, L.fmvd_plan_info_cuda_8
+        else:
+            raise ValueError("bits must be 8 or 16")
+        self._esink = ESINK_T(esink)
+        take_errors()
+        self.plan = init(self.h, self.w, _float_ptr_array(self.weights), _float_ptr_array(self.kernels),
+                         kh, kw, int(psf_type), self.nviews, int(nstreams), self._src, self._snk)
+        check()
+        if not self.plan:
+            raise FmvdError("plan creation failed")
+        if batch_planes:
+            self._set_batch(self.plan, int(batch_planes))
+
+    def info(self) -> dict:
+        d = (C.c_int * 6)()
+        self._info(self.plan, d)
+        return dict(data_h=d[0], data_w=d[1], fft_h=d[2], fft_w=d[3], views=d[4], batch=d[5])
+
+    def set_batch_planes(self, n: int) -> int:
+        return self._set_batch(self.plan, int(n))
+
+    def run(self, planes: Sequence[Sequence[np.ndarray]], iterations: int, want_estimate: bool = False):
+        """planes: sequence over z of per-view (dh, dw) sample arrays.
+        Returns (samples[z, dh, dw], estimate[z, dh, dw] or None)."""
+        self._planes_in = [[np.ascontiguousarray(v, dtype=self._dtype) for v in p] for p in planes]
+        self._planes_out, self._est_out, self._pos = [], [], 0
+        self._set_esink(self.plan, self._esink if want_estimate else ESINK_T(), None)
+        self._run(self.plan, int(iterations))
+        check()
+        out = np.stack(self._planes_out) if self._planes_out else np.zeros((0, self.h, self.w), self._dtype)
+        est = np.stack(self._est_out) if want_estimate and self._est_out else None
+        return out, est
+
+    def run_device(self, view_ptrs: Sequence[int], nplanes: int, iterations: int, out_ptr: int = 0,
+                   est_ptr: int = 0, init_ptr: int = 0, stream: int = 0) -> None:
+        """GPU-resident volume: raw device pointers (ints), enqueued on `stream`."""
+        arr = (C.c_void_p * self.nviews)(*[C.c_void_p(int(p)) for p in view_ptrs])
+        rc = self._volume(self.plan, arr, int(nplanes), int(iterations), C.c_void_p(init_ptr or None),
+                          C.c_void_p(out_ptr or None), C.c_void_p(est_ptr or None), C.c_void_p(stream or None))
+        check()
+        if rc != 0:
+            raise FmvdError("volume deconvolution failed")
+
+    def close(self):
+        if getattr(self, "plan", None):
+            self._destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def transform_to_file(volume: np.ndarray, inv: np.ndarray, target_shape, outfile: str,
+                      mask_file: Optional[str] = None, border: int = 0, zspacing: float = 1.0) -> None:
+    """Mirror of transform_cuda_{8,16}: volume[z, y, x] (uint8/uint16) -> raw file."""
+    L = lib()
+    vol = np.ascontiguousarray(volume)
+    d, h, w = vol.shape
+    td, th, tw = target_shape
+    if vol.dtype == np.uint16:
+        fn, ct = L.transform_cuda_16, C.c_uint16
+    elif vol.dtype == np.uint8:
+        fn, ct = L.transform_cuda_8, C.c_uint8
+    else:
+        raise TypeError("volume must be uint8 or uint16")
+    planes = (C.POINTER(ct) * d)(*[vol[z].ctypes.data_as(C.POINTER(ct)) for z in range(d)])
+    m = np.ascontiguousarray(inv, dtype=np.float32).reshape(12)
+    take_errors()
+    fn(planes, w, h, d, tw, th, td, m.ctypes.data_as(C.POINTER(C.c_float)), outfile.encode(),
+       1 if mask_file else 0, int(border), float(zspacing), mask_file.encode() if mask_file else None)
+    check()
+
+
+def transform_device(src_ptr: int, src_shape, dst_ptr: int, target_shape, inv: np.ndarray, bits: int = 16,
+                     stream: int = 0) -> None:
+    L = lib()
+    d, h, w = src_shape
+    td, th, tw = target_shape
+    m = np.ascontiguousarray(inv, dtype=np.float32).reshape(12)
+    fn = L.fmvd_transform_device_16 if bits == 16 else L.fmvd_transform_device_8
+    rc = fn(C.c_void_p(src_ptr), w, h, d, C.c_void_p(dst_ptr), tw, th, td,
+            m.ctypes.data_as(C.POINTER(C.c_float)), C.c_void_p(stream or None))
+    check()
+    if rc != 0:
+        raise FmvdError("transform failed")
+
+
+def mask_device(src_shape, target_shape, inv: np.ndarray, border: int, zspacing: float, dst_ptr: int,
+                stream: int = 0) -> None:
+    L = lib()
+    d, h, w = src_shape
+    td, th, tw = target_shape
+    m = np.ascontiguousarray(inv, dtype=np.float32).reshape(12)
+    rc = L.fmvd_transform_mask_device(w, h, d, tw, th, td, m.ctypes.data_as(C.POINTER(C.c_float)), int(border),
+                                      float(zspacing), C.c_void_p(dst_ptr), C.c_void_p(stream or None))
+    check()
+    if rc != 0:
+        raise FmvdError("mask failed")
+
+
+class KernelTiming:
+    """Per-kernel HIP-event timing recorded by the library around launches."""
+
+    def __enter__(self):
+        lib().fmvd_kernel_timing_reset()
+        lib().fmvd_kernel_timing_enable(1)
+        return self
+
+    def __exit__(self, *a):
+        lib().fmvd_kernel_timing_enable(0)
+
+    @staticmethod
+    def read(kind: str):
+        ms, n = C.c_double(), C.c_longlong()
+        lib().fmvd_kernel_timing_read(TIMING_KINDS[kind], C.byref(ms), C.byref(n))
+        return ms.value, n.value

@@ -1,0 +1,540 @@
+// Plane-wise multi-view RL engine: plan, batched device execution, plane
+// streaming and the C ABI of fmvd_deconvolve.h.
+//
+// Reference: fmvd_deconvolve.impl.cpp:42-336 (prepareKernels, prepareWeights,
+// fmvd_initialize_cuda, fmvd_deconvolve_planes_cuda, fmvd_destroy_cuda).
+//
+// Per view and iteration the device runs four launches over a whole batch of
+// planes (the reference runs eight per plane):
+//   A  column pass   XT = colconv(XE, K_v / H)                (spectrum of Q)
+//   B  row pass      XT = R2C(Q > 1 ? D_v / Q : Q), Q = C2R(XT)
+//   C  column pass   XT = colconv(XT, Khat_v / (H W))          (spectrum of C)
+//   D  row pass      E = C2R(XE), C = C2R(XT), E += w_v (E C - E), XE = R2C(E)/W
+// XE is the row spectrum of the estimate scaled by 1/W, so C2R(XE) = E.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fmvd_cuda_utils.h"
+#include "../../include/fmvd_deconvolve.h"
+#include "hip_check.hpp"
+#include "host_psf.hpp"
+#include "rl_launch.hpp"
+#include "timing.hpp"
+
+namespace fmvd {
+
+// ------------------------------------------------------------ RAII buffers
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    ~DeviceBuffer() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n)
+    {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) return;
+        FMVD_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct PinnedBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() { release(); }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n)
+    {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) return;
+        FMVD_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+        bytes = n;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { FMVD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream()
+    {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { FMVD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    ~Event()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+template <typename T> static void upload(DeviceBuffer& b, const std::vector<T>& v)
+{
+    b.alloc(sizeof(T) * v.size());
+    FMVD_HIP(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+static size_t env_bytes(const char* name, size_t dflt)
+{
+    const char* s = getenv(name);
+    if (!s || !*s) return dflt;
+    return (size_t)strtoull(s, nullptr, 10);
+}
+
+// ------------------------------------------------------------------ plan
+template <typename S> struct Plan {
+    using source_t = int (*)(S**, int);
+    using sink_t = void (*)(S*);
+
+    int dataH = 0, dataW = 0, kernelH = 0, kernelW = 0, nViews = 0, nStreams = 0;
+    int fftH = 0, fftW = 0, N = 0;
+    int device = 0;
+    source_t source = nullptr;
+    sink_t sink = nullptr;
+    fmvd_estimate_sink_t esink = nullptr;
+    void* euser = nullptr;
+    int batchCap = 0;
+
+    DeviceBuffer twN, twW, twH;
+    std::vector<DeviceBuffer> K, Kh, Wn;
+    DeviceBuffer XE, XT;
+    std::unique_ptr<Stream> compute, copy;
+
+    size_t plane() const { return (size_t)dataH * dataW; }
+    size_t xplane() const { return (size_t)N * fftH; }  // complex per plane
+
+    RowGeom row_geom() const
+    {
+        RowGeom g;
+        g.H = fftH;
+        g.W = fftW;
+        g.N = N;
+        g.dataH = dataH;
+        g.dataW = dataW;
+        // data pad uses swapped half-widths (fmvd_deconvolve.impl.cpp:36-37
+        // passes (kernelH, kernelW) into parameters named (kernelW, kernelH),
+        // fmvd_deconvolve_cuda.impl.cu:76-77,87-88)
+        g.dkY = kernelW / 2;
+        g.dkX = kernelH / 2;
+        g.wkY = kernelH / 2;
+        g.wkX = kernelW / 2;
+        g.twN = twN.as<float2>();
+        g.twW = twW.as<float2>();
+        return g;
+    }
+    ColGeom col_geom() const
+    {
+        ColGeom g;
+        g.H = fftH;
+        g.N = N;
+        g.twH = twH.as<float2>();
+        return g;
+    }
+
+    int batch_planes() const
+    {
+        if (batchCap > 0) return batchCap;
+        const size_t budget = env_bytes("FMVD_BATCH_BYTES", (size_t)2 << 30);
+        const size_t per = 2 * xplane() * sizeof(float2);
+        size_t nb = budget / (per ? per : 1);
+        nb = std::max<size_t>(1, std::min<size_t>(nb, 4096));
+        return (int)std::max<size_t>(nb, (size_t)std::max(1, nStreams));
+    }
+
+    void ensure_work(int planes)
+    {
+        XE.alloc(xplane() * sizeof(float2) * (size_t)planes);
+        XT.alloc(xplane() * sizeof(float2) * (size_t)planes);
+    }
+
+    void init(int dH, int dW, const float* const* h_Weights, float** h_Kernel, int kH, int kW,
+              int psf, int nV, int nS, source_t src, sink_t snk)
+    {
+        if (dH < 1 || dW < 1 || kH < 1 || kW < 1 || nV < 1)
+            throw_msg("invalid plan dimensions (dataH, dataW, kernelH, kernelW, nViews must be >= 1)");
+        if (psf < 0 || psf > 3) throw_msg("invalid PSF type (expected 0..3)");
+        if (!h_Weights || !h_Kernel) throw_msg("weights and kernels must not be NULL");
+        dataH = dH;
+        dataW = dW;
+        kernelH = kH;
+        kernelW = kW;
+        nViews = nV;
+        nStreams = nS < 1 ? 1 : nS;
+        source = src;
+        sink = snk;
+        fftH = snap_transform_size(dataH + kernelH - 1);
+        fftW = snap_transform_size(dataW + kernelW - 1);
+        N = fftW / 2;
+        if (!row_size_supported(N) || !col_size_supported(fftH) || fftW < 32 || fftH < 16) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "unsupported FFT extent %d x %d (data %d x %d, kernel %d x %d)",
+                     fftH, fftW, dataH, dataW, kernelH, kernelW);
+            throw_msg(msg);
+        }
+        FMVD_HIP(hipGetDevice(&device));
+        compute.reset(new Stream());
+        copy.reset(new Stream());
+
+        // fmvd_deconvolve.impl.cpp:61-64: kernel-hat from the raw kernels,
+        // then the caller's kernels are normalised in place.
+        const int len = kernelH * kernelW;
+        std::vector<double> khat = compute_kernel_hat(h_Kernel, kernelW, kernelH, nViews, psf);
+        for (int v = 0; v < nViews; ++v) normalize_kernel_f32(h_Kernel[v], len);
+
+        upload(twN, twiddles(N, N));
+        upload(twW, twiddles(fftW, N));
+        upload(twH, twiddles(fftH, fftH));
+
+        K.resize(nViews);
+        Kh.resize(nViews);
+        Wn.resize(nViews);
+        std::vector<double> kv(len);
+        for (int v = 0; v < nViews; ++v) {
+            for (int i = 0; i < len; ++i) kv[i] = (double)h_Kernel[v][i];
+            // pass A consumes XE = R2C(E)/W: fold 1/H here; pass C consumes
+            // R2C(R): fold 1/(H W) (modulateAndNormalize's c, fmvd_deconvolve_cuda.cu:369)
+            upload(K[v], kernel_spectrum_transposed(kv.data(), kernelW, kernelH, fftH, fftW, 1.0 / fftH));
+            upload(Kh[v], kernel_spectrum_transposed(khat.data() + (size_t)v * len, kernelW, kernelH,
+                                                     fftH, fftW, 1.0 / ((double)fftH * fftW)));
+        }
+        std::vector<float> wn = normalized_weights(h_Weights, dataW, dataH, nViews);
+        for (int v = 0; v < nViews; ++v) {
+            std::vector<float> one(wn.begin() + (size_t)v * plane(), wn.begin() + (size_t)(v + 1) * plane());
+            upload(Wn[v], one);
+        }
+    }
+
+    // RL on `np` resident planes; views[v] points at the first plane of view v.
+    void run_batch(const S* const* views, long dstride, int np, int iterations, const float* init,
+                   long istride, S* out, float* est, long ostride, hipStream_t s)
+    {
+        ensure_work(np);
+        const RowGeom rg = row_geom();
+        const ColGeom cg = col_geom();
+        const long xs = (long)xplane();
+        float2* xe = XE.as<float2>();
+        float2* xt = XT.as<float2>();
+        {
+            TimedScope ts(kTimeIngest, s);
+            if (init) launch_row_init(rg, xe, xs, init, istride, np, s);
+            else launch_const_init(rg, xe, xs, 10.0f * (float)nViews, np, s);
+        }
+        for (int it = 0; it < iterations; ++it) {
+            for (int v = 0; v < nViews; ++v) {
+                {
+                    TimedScope ts(kTimeColumn, s);
+                    launch_col_conv(cg, xe, xt, xs, K[v].as<float2>(), np, s);
+                }
+                {
+                    TimedScope ts(kTimeRatio, s);
+                    launch_row_ratio<S>(rg, xt, xs, views[v], dstride, np, s);
+                }
+                {
+                    TimedScope ts(kTimeColumn, s);
+                    launch_col_conv(cg, xt, xt, xs, Kh[v].as<float2>(), np, s);
+                }
+                {
+                    TimedScope ts(kTimeUpdate, s);
+                    launch_row_update(rg, xe, xt, xs, Wn[v].as<float>(), np, s);
+                }
+            }
+        }
+        TimedScope ts(kTimeEgress, s);
+        launch_row_final<S>(rg, xe, xs, out, est, ostride, np, s);
+    }
+
+    void run_volume(const S* const* dviews, int nplanes, int iterations, const float* init, S* out,
+                    float* est, hipStream_t s)
+    {
+        if (nplanes < 0 || iterations < 0) throw_msg("negative plane or iteration count");
+        if (!dviews) throw_msg("d_views must not be NULL");
+        const int nb = batch_planes();
+        const size_t pl = plane();
+        std::vector<const S*> vv(nViews);
+        for (int start = 0; start < nplanes; start += nb) {
+            const int n = std::min(nb, nplanes - start);
+            for (int v = 0; v < nViews; ++v) vv[v] = dviews[v] + (size_t)start * pl;
+            run_batch(vv.data(), (long)pl, n, iterations, init ? init + (size_t)start * pl : nullptr,
+                      (long)pl, out ? out + (size_t)start * pl : nullptr,
+                      est ? est + (size_t)start * pl : nullptr, (long)pl, s);
+        }
+    }
+
+    // fmvd_deconvolve.impl.cpp:205-300, as a double-buffered batch pipeline:
+    // while batch i computes on the compute stream, the host pulls batch i+1
+    // from the datasource and uploads it on the copy stream, and pushes batch
+    // i-1's results to the datasink.  Planes are pulled and pushed in z order.
+    void deconvolve_planes(int iterations)
+    {
+        if (!source || !sink) throw_msg("datasource and datasink must be set");
+        if (iterations < 0) throw_msg("negative iteration count");
+        FMVD_HIP(hipSetDevice(device));
+        const int nb = batch_planes();
+        const size_t pl = plane();
+        const size_t inBytes = sizeof(S) * pl * nViews * (size_t)nb;
+        DeviceBuffer din[2], dout, dest;
+        PinnedBuffer hin[2], hout[2], hest[2];
+        Event up[2], done[2];
+        for (int k = 0; k < 2; ++k) {
+            din[k].alloc(inBytes);
+            hin[k].alloc(inBytes);
+            hout[k].alloc(sizeof(S) * pl * nb);
+            if (esink) hest[k].alloc(sizeof(float) * pl * nb);
+        }
+        dout.alloc(sizeof(S) * pl * nb);
+        if (esink) dest.alloc(sizeof(float) * pl * nb);
+        ensure_work(nb);
+
+        std::vector<S*> hviews[2];
+        for (int k = 0; k < 2; ++k)
+            for (int v = 0; v < nViews; ++v) hviews[k].push_back(hin[k].as<S>() + (size_t)v * nb * pl);
+        bool exhausted = false;
+        auto fill = [&](int slot) -> int {
+            int n = 0;
+            while (n < nb && !exhausted) {
+                if (!source(hviews[slot].data(), (int)(n * pl))) {
+                    exhausted = true;
+                    break;
+                }
+                ++n;
+            }
+            return n;
+        };
+        auto send = [&](int slot, int n) {
+            for (int v = 0; v < nViews; ++v)
+                FMVD_HIP(hipMemcpyAsync(din[slot].as<S>() + (size_t)v * nb * pl, hviews[slot][v],
+                                        sizeof(S) * pl * n, hipMemcpyHostToDevice, copy->s));
+            FMVD_HIP(hipEventRecord(up[slot].e, copy->s));
+        };
+        int zout = 0;
+        auto drain = [&](int slot, int n) {
+            FMVD_HIP(hipEventSynchronize(done[slot].e));
+            for (int j = 0; j < n; ++j) {
+                if (esink) esink(hest[slot].as<float>() + (size_t)j * pl, zout, euser);
+                sink(hout[slot].as<S>() + (size_t)j * pl);
+                ++zout;
+            }
+        };
+
+        int cnt[2] = {0, 0};
+        int slot = 0;
+        cnt[0] = fill(0);
+        if (cnt[0] == 0) return;
+        send(0, cnt[0]);
+        bool prev = false;
+        std::vector<const S*> vv(nViews);
+        for (;;) {
+            hipStream_t cs = compute->s;
+            FMVD_HIP(hipStreamWaitEvent(cs, up[slot].e, 0));
+            for (int v = 0; v < nViews; ++v) vv[v] = din[slot].as<S>() + (size_t)v * nb * pl;
+            run_batch(vv.data(), (long)pl, cnt[slot], iterations, nullptr, 0, dout.as<S>(),
+                      esink ? dest.as<float>() : nullptr, (long)pl, cs);
+            FMVD_HIP(hipMemcpyAsync(hout[slot].p, dout.p, sizeof(S) * pl * cnt[slot],
+                                    hipMemcpyDeviceToHost, cs));
+            if (esink)
+                FMVD_HIP(hipMemcpyAsync(hest[slot].p, dest.p, sizeof(float) * pl * cnt[slot],
+                                        hipMemcpyDeviceToHost, cs));
+            FMVD_HIP(hipEventRecord(done[slot].e, cs));
+            const int ns = slot ^ 1;
+            if (prev) drain(ns, cnt[ns]);
+            cnt[ns] = exhausted ? 0 : fill(ns);
+            if (cnt[ns] > 0) send(ns, cnt[ns]);
+            if (cnt[ns] == 0) {
+                drain(slot, cnt[slot]);
+                break;
+            }
+            prev = true;
+            slot = ns;
+        }
+    }
+};
+
+}  // namespace fmvd
+
+// ---------------------------------------------------------------- C ABI
+struct fmvd_plan_cuda_16 {
+    fmvd::Plan<uint16_t> impl;
+};
+struct fmvd_plan_cuda_8 {
+    fmvd::Plan<uint8_t> impl;
+};
+
+namespace {
+
+template <typename PlanT, typename S, typename Src, typename Snk>
+PlanT* initialize(int dataH, int dataW, float const* const* h_Weights, float** h_Kernel, int kernelH,
+                  int kernelW, fmvd_psf_type t, int nViews, int nstreams, Src src, Snk snk)
+{
+    FMVD_API_GUARD(nullptr, {
+        std::unique_ptr<PlanT> p(new PlanT());
+        p->impl.init(dataH, dataW, h_Weights, h_Kernel, kernelH, kernelW, (int)t, nViews, nstreams,
+                     reinterpret_cast<int (*)(S**, int)>(src), reinterpret_cast<void (*)(S*)>(snk));
+        return p.release();
+    })
+}
+
+template <typename PlanT> int plan_info(const PlanT* p, int* dims)
+{
+    if (!p || !dims) return -1;
+    dims[0] = p->impl.dataH;
+    dims[1] = p->impl.dataW;
+    dims[2] = p->impl.fftH;
+    dims[3] = p->impl.fftW;
+    dims[4] = p->impl.nViews;
+    dims[5] = p->impl.batch_planes();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+struct fmvd_plan_cuda_16* fmvd_initialize_cuda_16(int dataH, int dataW, float const* const* h_Weights,
+                                                  float** h_Kernel, int kernelH, int kernelW,
+                                                  fmvd_psf_type iteration_type, int nViews,
+                                                  int nstreams, datasource_t_16 get_next_plane,
+                                                  datasink_t_16 return_next_plane)
+{
+    return initialize<fmvd_plan_cuda_16, uint16_t>(dataH, dataW, h_Weights, h_Kernel, kernelH,
+                                                   kernelW, iteration_type, nViews, nstreams,
+                                                   get_next_plane, return_next_plane);
+}
+
+struct fmvd_plan_cuda_8* fmvd_initialize_cuda_8(int dataH, int dataW, float const* const* h_Weights,
+                                                float** h_Kernel, int kernelH, int kernelW,
+                                                fmvd_psf_type iteration_type, int nViews,
+                                                int nstreams, datasource_t_8 get_next_plane,
+                                                datasink_t_8 return_next_plane)
+{
+    return initialize<fmvd_plan_cuda_8, uint8_t>(dataH, dataW, h_Weights, h_Kernel, kernelH,
+                                                 kernelW, iteration_type, nViews, nstreams,
+                                                 get_next_plane, return_next_plane);
+}
+
+void fmvd_deconvolve_planes_cuda_16(const struct fmvd_plan_cuda_16* plan, int iterations)
+{
+    FMVD_API_GUARD(, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_16*>(plan)->impl.deconvolve_planes(iterations);
+    })
+}
+
+void fmvd_deconvolve_planes_cuda_8(const struct fmvd_plan_cuda_8* plan, int iterations)
+{
+    FMVD_API_GUARD(, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_8*>(plan)->impl.deconvolve_planes(iterations);
+    })
+}
+
+void fmvd_destroy_cuda_16(struct fmvd_plan_cuda_16* plan)
+{
+    if (plan) (void)hipDeviceSynchronize();
+    delete plan;
+}
+
+void fmvd_destroy_cuda_8(struct fmvd_plan_cuda_8* plan)
+{
+    if (plan) (void)hipDeviceSynchronize();
+    delete plan;
+}
+
+int fmvd_set_estimate_sink_cuda_16(struct fmvd_plan_cuda_16* plan, fmvd_estimate_sink_t sink, void* user)
+{
+    if (!plan) return -1;
+    plan->impl.esink = sink;
+    plan->impl.euser = user;
+    return 0;
+}
+
+int fmvd_set_estimate_sink_cuda_8(struct fmvd_plan_cuda_8* plan, fmvd_estimate_sink_t sink, void* user)
+{
+    if (!plan) return -1;
+    plan->impl.esink = sink;
+    plan->impl.euser = user;
+    return 0;
+}
+
+int fmvd_set_batch_planes_cuda_16(struct fmvd_plan_cuda_16* plan, int planes)
+{
+    if (!plan) return -1;
+    plan->impl.batchCap = planes > 0 ? planes : 0;
+    return plan->impl.batch_planes();
+}
+
+int fmvd_set_batch_planes_cuda_8(struct fmvd_plan_cuda_8* plan, int planes)
+{
+    if (!plan) return -1;
+    plan->impl.batchCap = planes > 0 ? planes : 0;
+    return plan->impl.batch_planes();
+}
+
+int fmvd_deconvolve_volume_device_cuda_16(const struct fmvd_plan_cuda_16* plan,
+                                          const unsigned short* const* d_views, int nPlanes,
+                                          int iterations, const float* d_init, unsigned short* d_out,
+                                          float* d_estimate, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_16*>(plan)->impl.run_volume(d_views, nPlanes, iterations, d_init, d_out,
+                                                              d_estimate, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_deconvolve_volume_device_cuda_8(const struct fmvd_plan_cuda_8* plan,
+                                         const unsigned char* const* d_views, int nPlanes,
+                                         int iterations, const float* d_init, unsigned char* d_out,
+                                         float* d_estimate, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_8*>(plan)->impl.run_volume(d_views, nPlanes, iterations, d_init, d_out,
+                                                             d_estimate, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_plan_info_cuda_16(const struct fmvd_plan_cuda_16* plan, int* dims) { return plan_info(plan, dims); }
+int fmvd_plan_info_cuda_8(const struct fmvd_plan_cuda_8* plan, int* dims) { return plan_info(plan, dims); }
+
+int fmvd_snap_transform_size(int n) { return fmvd::snap_transform_size(n); }
+
+int fmvd_fft_extent_supported(int fftH, int fftW)
+{
+    return fftW >= 32 && fftH >= 16 && (fftW % 2 == 0) && fmvd::row_size_supported(fftW / 2) &&
+           fmvd::col_size_supported(fftH);
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Launch interface between the RL host engine and the RL device kernels.
+//
+// Device layout of one plane's "row spectrum" (transposed, packed):
+//   X[kx][y], kx in [0, N) with N = fftW/2, y in [0, fftH), complex64,
+//   X[kx][y] = sum_x f[y][x] exp(-2 pi i kx x / fftW)   for kx in [1, N),
+//   X[0][y]  = (DC(y), Nyquist(y))  packed as (re, im),
+// i.e. one plane occupies N*fftH complex = fftW*fftH floats.  The column pass
+// transforms each kx-row (contiguous, length fftH); the row passes read and
+// write it transposed in tiles of G rows.
+//
+// Kernel spectra use the same transposed layout with N+1 rows: rows 1..N-1
+// hold K[ky][kx]; row 0 holds K[ky][0] and row N holds K[ky][N] (both
+// complex), already multiplied by the pass's normalisation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fmvd {
+
+struct RowGeom {
+    int H, W, N;          // fftH, fftW, fftW/2
+    int dataH, dataW;
+    int dkY, dkX;         // data pad half-widths (kernelW/2, kernelH/2: swapped as the reference)
+    int wkY, wkX;         // weight pad half-widths (kernelH/2, kernelW/2)
+    const float2* twN;    // exp(-2 pi i m / N), m < N
+    const float2* twW;    // exp(-2 pi i p / W), p < N
+};
+
+struct ColGeom {
+    int H, N;
+    const float2* twH;    // exp(-2 pi i m / H), m < H
+};
+
+// Pass B: X <- R2C(ratio(D, C2R(X))) for nplanes planes, in place.
+// data: first plane of this view; dstride: elements between planes.
+template <typename SAMPLE>
+void launch_row_ratio(const RowGeom& g, float2* X, long xstride, const SAMPLE* data, long dstride,
+                      int nplanes, hipStream_t s);
+// Pass D: E = C2R(XE); C = C2R(XC); E += w*(E*C - E); XE <- R2C(E)/W.
+void launch_row_update(const RowGeom& g, float2* XE, const float2* XC, long xstride,
+                       const float* weights, int nplanes, hipStream_t s);
+// Egress: E = C2R(XE) on rows < dataH; out = sample(E), est = E (either may be null).
+template <typename SAMPLE>
+void launch_row_final(const RowGeom& g, const float2* XE, long xstride, SAMPLE* out, float* est,
+                      long ostride, int nplanes, hipStream_t s);
+// Warm-start ingest: XE <- R2C(pad(E0))/W, E0 is dataH x dataW per plane.
+void launch_row_init(const RowGeom& g, float2* XE, long xstride, const float* e0, long estride,
+                     int nplanes, hipStream_t s);
+// Constant start E0 = value everywhere: XE[0][y] = (value, 0), rest 0.
+void launch_const_init(const RowGeom& g, float2* XE, long xstride, float value, int nplanes,
+                       hipStream_t s);
+// Passes A / C: Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]) for every kx.
+void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride,
+                     const float2* K, int nplanes, hipStream_t s);
+
+bool row_size_supported(int N);
+bool col_size_supported(int H);
+
+}  // namespace fmvd

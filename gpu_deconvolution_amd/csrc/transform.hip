@@ -1,0 +1,288 @@
+// Affine view transform and border-weight mask (gfx950).
+//
+// Reference: fmvd_transform.impl.cu:24-56 (transform_plane_kernel, a CUDA
+// 3D-texture trilinear fetch at r + 0.5 with border address mode and
+// normalised-float read), :77-224 (transform_cuda driver) and
+// fmvd_transform.cu:19-89 (create_transformed_mask).
+//
+// Arithmetic: source coordinates and interpolation weights are evaluated in
+// FP64 with a fixed operation order and no contraction, so every output
+// sample equals the FP64 definition bit for bit:
+//   r = ((m0 x + m1 y) + m2 z) + m3 per axis, i = floor(r), a = r - i,
+//   nested lerps v*(1-a) + v'*a in x, then y, then z (texels outside -> 0),
+//   out = clamp(floor(interp / (2^B-1) * 2^B + 0.5), 0, 2^B-1).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/fmvd_transform.h"
+#include "hip_check.hpp"
+#include "timing.hpp"
+
+namespace fmvd {
+
+struct Mat12 {
+    double m[12];
+};
+
+template <typename S>
+__device__ __forceinline__ double texel(const S* __restrict__ src, int w, int h, int d, int x, int y,
+                                        int z)
+{
+    if (x < 0 || y < 0 || z < 0 || x >= w || y >= h || z >= d) return 0.0;
+    return (double)src[((long)z * h + y) * w + x];
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ src, int w, int h, int d,
+                                                        S* __restrict__ dst, int tw, int th, int z0,
+                                                        Mat12 M)
+{
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int zl = blockIdx.z;
+    const int z = z0 + zl;
+    if (x >= tw || y >= th) return;
+    const double xd = (double)x, yd = (double)y, zd = (double)z;
+    const double rx = ((M.m[0] * xd + M.m[1] * yd) + M.m[2] * zd) + M.m[3];
+    const double ry = ((M.m[4] * xd + M.m[5] * yd) + M.m[6] * zd) + M.m[7];
+    const double rz = ((M.m[8] * xd + M.m[9] * yd) + M.m[10] * zd) + M.m[11];
+    double val = 0.0;
+    if (rx > -2.0 && ry > -2.0 && rz > -2.0 && rx < (double)w + 1.0 && ry < (double)h + 1.0 &&
+        rz < (double)d + 1.0) {
+        const double fx = floor(rx), fy = floor(ry), fz = floor(rz);
+        const double ax = rx - fx, ay = ry - fy, az = rz - fz;
+        const double bx = 1.0 - ax, by = 1.0 - ay, bz = 1.0 - az;
+        const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+        const double c00 = texel(src, w, h, d, ix, iy, iz) * bx + texel(src, w, h, d, ix + 1, iy, iz) * ax;
+        const double c10 = texel(src, w, h, d, ix, iy + 1, iz) * bx + texel(src, w, h, d, ix + 1, iy + 1, iz) * ax;
+        const double c01 = texel(src, w, h, d, ix, iy, iz + 1) * bx + texel(src, w, h, d, ix + 1, iy, iz + 1) * ax;
+        const double c11 = texel(src, w, h, d, ix, iy + 1, iz + 1) * bx + texel(src, w, h, d, ix + 1, iy + 1, iz + 1) * ax;
+        const double c0 = c00 * by + c10 * ay;
+        const double c1 = c01 * by + c11 * ay;
+        val = c0 * bz + c1 * az;
+    }
+    constexpr double maxv = (double)((1u << (8 * sizeof(S))) - 1u);
+    constexpr double scale = (double)(1u << (8 * sizeof(S)));
+    const double yv = (val / maxv) * scale + 0.5;
+    S o;
+    if (yv >= maxv) o = (S)maxv;
+    else if (yv > 0.0) o = (S)floor(yv);
+    else o = (S)0;
+    dst[((long)zl * th + y) * tw + x] = o;
+}
+
+__global__ __launch_bounds__(256) void mask_kernel(float* __restrict__ out, int z, int w, int h, int d,
+                                                   double zspacing, int tw, int th, double border,
+                                                   Mat12 M)
+{
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= tw || y >= th) return;
+    const double xd = (double)x, yd = (double)y, zd = (double)z;
+    const double rx = ((M.m[0] * xd + M.m[1] * yd) + M.m[2] * zd) + M.m[3];
+    const double ry = ((M.m[4] * xd + M.m[5] * yd) + M.m[6] * zd) + M.m[7];
+    const double rz = ((M.m[8] * xd + M.m[9] * yd) + M.m[10] * zd) + M.m[11];
+    float r = 0.0f;
+    if (!(rx < 0 || rx >= w || ry < 0 || ry >= h || rz < 0 || rz >= d)) {
+        // fmvd_transform.cu:46-64 (w/2 etc. are integer divisions)
+        double dx = rx < (double)(w / 2) ? rx : (double)w - rx;
+        double dy = ry < (double)(h / 2) ? ry : (double)h - ry;
+        double dz = rz < (double)(d / 2) ? rz : (double)d - rz;
+        dz *= zspacing;
+        double v = 1.0;
+        if (dx < border) v *= 0.5 * (1.0 - cos(dx / border * M_PI));
+        if (dy < border) v *= 0.5 * (1.0 - cos(dy / border * M_PI));
+        if (dz < border) v *= 0.5 * (1.0 - cos(dz / border * M_PI));
+        r = (float)v;
+    }
+    out[(long)y * tw + x] = r;
+}
+
+static Mat12 to_mat(const float* m)
+{
+    Mat12 M;
+    for (int i = 0; i < 12; ++i) M.m[i] = (double)m[i];
+    return M;
+}
+
+template <typename S>
+static void launch_transform(const S* src, int w, int h, int d, S* dst, int tw, int th, int z0, int nz,
+                             const Mat12& M, hipStream_t s)
+{
+    if (nz <= 0 || tw <= 0 || th <= 0) return;
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((tw + 63) / 64), (unsigned)((th + 3) / 4), (unsigned)nz);
+    TimedScope ts(kTimeTransform, s);
+    transform_kernel<S><<<grid, block, 0, s>>>(src, w, h, d, dst, tw, th, z0, M);
+    FMVD_LAUNCH_CHECK();
+}
+
+static void launch_mask(float* out, int w, int h, int d, int tw, int th, int td, const Mat12& M,
+                        int border, float zspacing, hipStream_t s)
+{
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((tw + 63) / 64), (unsigned)((th + 3) / 4));
+    mask_kernel<<<grid, block, 0, s>>>(out, td / 2, w, h, d, (double)zspacing, tw, th, (double)border, M);
+    FMVD_LAUNCH_CHECK();
+}
+
+static void check_dims(int w, int h, int d, int tw, int th, int td)
+{
+    if (w < 1 || h < 1 || d < 1 || tw < 1 || th < 1 || td < 1)
+        throw_msg("invalid transform dimensions (all extents must be >= 1)");
+}
+
+// fmvd_transform.impl.cu:77-224: whole source volume resident on the device,
+// target planes computed in chunks and written to `outfile` in z order while
+// the next chunk computes.
+template <typename S>
+static void transform_files(S** h_data, int w, int h, int d, int tw, int th, int td, const float* inv,
+                            const char* outfile, int createMasks, int border, float zspacing,
+                            const char* maskfile)
+{
+    check_dims(w, h, d, tw, th, td);
+    if (!h_data || !inv || !outfile) throw_msg("transform: NULL argument");
+    const Mat12 M = to_mat(inv);
+    const size_t plane = (size_t)w * h, tplane = (size_t)tw * th;
+    S* dsrc = nullptr;
+    S* ddst = nullptr;
+    S* hdst = nullptr;
+    float* dmask = nullptr;
+    float* hmask = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    FILE* out = nullptr;
+    auto cleanup = [&] {
+        if (s) (void)hipStreamSynchronize(s);
+        if (dsrc) (void)hipFree(dsrc);
+        if (ddst) (void)hipFree(ddst);
+        if (hdst) (void)hipHostFree(hdst);
+        if (dmask) (void)hipFree(dmask);
+        if (hmask) (void)hipHostFree(hmask);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+        if (out) fclose(out);
+    };
+    try {
+        FMVD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        FMVD_HIP(hipMalloc(&dsrc, sizeof(S) * plane * d));
+        for (int z = 0; z < d; ++z)
+            FMVD_HIP(hipMemcpy(dsrc + plane * z, h_data[z], sizeof(S) * plane, hipMemcpyHostToDevice));
+        if (createMasks) {
+            if (!maskfile) throw_msg("transform: mask requested without a mask file");
+            FMVD_HIP(hipMalloc(&dmask, sizeof(float) * tplane));
+            FMVD_HIP(hipHostMalloc(&hmask, sizeof(float) * tplane, hipHostMallocDefault));
+            launch_mask(dmask, w, h, d, tw, th, td, M, border, zspacing, s);
+            FMVD_HIP(hipMemcpyAsync(hmask, dmask, sizeof(float) * tplane, hipMemcpyDeviceToHost, s));
+            FMVD_HIP(hipStreamSynchronize(s));
+            FILE* mf = fopen(maskfile, "wb");
+            if (!mf) throw_msg(std::string("cannot open mask file for writing: ") + maskfile);
+            const size_t wr = fwrite(hmask, sizeof(float), tplane, mf);
+            fclose(mf);
+            if (wr != tplane) throw_msg("short write to mask file");
+        }
+        out = fopen(outfile, "wb");
+        if (!out) throw_msg(std::string("cannot open output file for writing: ") + outfile);
+        // chunk of target planes per launch (~64 MiB of output)
+        int zc = (int)std::max<size_t>(1, ((size_t)64 << 20) / (sizeof(S) * tplane));
+        zc = std::min(zc, td);
+        FMVD_HIP(hipMalloc(&ddst, sizeof(S) * tplane * zc * 2));
+        FMVD_HIP(hipHostMalloc(&hdst, sizeof(S) * tplane * zc * 2, hipHostMallocDefault));
+        for (auto& e : ev) FMVD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        int pend_z = -1, pend_n = 0, pend_slot = 0;
+        auto flush = [&](int slot, int n) {
+            FMVD_HIP(hipEventSynchronize(ev[slot]));
+            const size_t cnt = tplane * n;
+            if (fwrite(hdst + tplane * zc * slot, sizeof(S), cnt, out) != cnt)
+                throw_msg("short write to output file");
+        };
+        int slot = 0;
+        for (int z0 = 0; z0 < td; z0 += zc) {
+            const int n = std::min(zc, td - z0);
+            S* dd = ddst + tplane * zc * slot;
+            launch_transform<S>(dsrc, w, h, d, dd, tw, th, z0, n, M, s);
+            FMVD_HIP(hipMemcpyAsync(hdst + tplane * zc * slot, dd, sizeof(S) * tplane * n,
+                                    hipMemcpyDeviceToHost, s));
+            FMVD_HIP(hipEventRecord(ev[slot], s));
+            if (pend_z >= 0) flush(pend_slot, pend_n);
+            pend_z = z0;
+            pend_n = n;
+            pend_slot = slot;
+            slot ^= 1;
+        }
+        if (pend_z >= 0) flush(pend_slot, pend_n);
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+}  // namespace fmvd
+
+extern "C" {
+
+void transform_cuda_8(unsigned char** h_data, int w, int h, int d, int tw, int th, int td,
+                      float* h_inverse, const char* outfile, int createTransformedMasks, int border,
+                      float zspacing, const char* maskfile)
+{
+    FMVD_API_GUARD(, {
+        fmvd::transform_files<uint8_t>(h_data, w, h, d, tw, th, td, h_inverse, outfile,
+                                       createTransformedMasks, border, zspacing, maskfile);
+    })
+}
+
+void transform_cuda_16(unsigned short** h_data, int w, int h, int d, int tw, int th, int td,
+                       float* h_inverse, const char* outfile, int createTransformedMasks, int border,
+                       float zspacing, const char* maskfile)
+{
+    FMVD_API_GUARD(, {
+        fmvd::transform_files<uint16_t>(h_data, w, h, d, tw, th, td, h_inverse, outfile,
+                                        createTransformedMasks, border, zspacing, maskfile);
+    })
+}
+
+int fmvd_transform_device_8(const unsigned char* d_src, int w, int h, int d, unsigned char* d_dst,
+                            int tw, int th, int td, const float* h_inverse, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        fmvd::check_dims(w, h, d, tw, th, td);
+        if (!d_src || !d_dst || !h_inverse) fmvd::throw_msg("transform: NULL argument");
+        fmvd::launch_transform<uint8_t>(d_src, w, h, d, d_dst, tw, th, 0, td, fmvd::to_mat(h_inverse),
+                                        (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_transform_device_16(const unsigned short* d_src, int w, int h, int d, unsigned short* d_dst,
+                             int tw, int th, int td, const float* h_inverse, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        fmvd::check_dims(w, h, d, tw, th, td);
+        if (!d_src || !d_dst || !h_inverse) fmvd::throw_msg("transform: NULL argument");
+        fmvd::launch_transform<uint16_t>(d_src, w, h, d, d_dst, tw, th, 0, td, fmvd::to_mat(h_inverse),
+                                         (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_transform_mask_device(int w, int h, int d, int tw, int th, int td, const float* h_inverse,
+                               int border, float zspacing, float* d_mask, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        fmvd::check_dims(w, h, d, tw, th, td);
+        if (!d_mask || !h_inverse) fmvd::throw_msg("mask: NULL argument");
+        fmvd::launch_mask(d_mask, w, h, d, tw, th, td, fmvd::to_mat(h_inverse), border, zspacing,
+                          (hipStream_t)stream);
+        return 0;
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+/*
+ * Plane-wise multi-view Richardson-Lucy deconvolution: C ABI of
+ * libNativeSPIMReconstructionCuda.so (MI355X / gfx950 build).
+ *
+ * Drop-in for the reference interface declared in
+ *   fmvd_deconvolve.h:9-15        (fmvd_psf_type, fmvd_malloc, fmvd_free)
+ *   fmvd_deconvolve.impl.h:6-66   (datasource/datasink, plan, init/process/
+ *                                  destroy, files front end)
+ * of bene51/gpu_deconvolution.  The names keep the reference's "_cuda"
+ * suffix so existing callers (the Fiji plugin through JNI, the CLI) link
+ * unchanged; the functions get C linkage here (the reference compiled them
+ * with C++ linkage).  The plan struct is opaque: callers only ever held a
+ * pointer to it.
+ *
+ * Errors: every function reports failures once through the handler installed
+ * with setErrorHandler() (fmvd_cuda_utils.h; default prints and exits, as the
+ * reference does), releases what the failed call allocated and returns
+ * (NULL from the initialisers, a negative value from the int-returning
+ * extensions).
+ */
+#ifndef FMVD_DECONVOLVE_H
+#define FMVD_DECONVOLVE_H
+
+#include <stddef.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fmvd_deconvolve.h:9 */
+enum fmvd_psf_type { independent, efficient_bayesian, optimization_1, optimization_2 };
+typedef enum fmvd_psf_type fmvd_psf_type;
+
+/* fmvd_deconvolve.h:11-15: page-locked host memory. */
+void *fmvd_malloc(size_t size);
+void fmvd_free(void *p);
+
+/* fmvd_deconvolve.impl.h:6-8.  A datasource fills buffer[v] + offset with the
+ * next plane of every view (dataW*dataH samples, x fastest) and returns 0 when
+ * there is no more data.  A datasink consumes one result plane synchronously
+ * (the buffer is reused after it returns).  Planes are requested and returned
+ * in z order on the calling thread. */
+typedef int (*datasource_t_16)(unsigned short **buffer, int offset);
+typedef void (*datasink_t_16)(unsigned short *buffer);
+typedef int (*datasource_t_8)(unsigned char **buffer, int offset);
+typedef void (*datasink_t_8)(unsigned char *buffer);
+
+struct fmvd_plan_cuda_16; /* opaque (fmvd_deconvolve.impl.h:10-37) */
+struct fmvd_plan_cuda_8;
+
+/* fmvd_deconvolve.impl.h:39-48 / fmvd_deconvolve.impl.cpp:130-203.
+ * Normalises h_Kernel[v] in place (as the reference does), copies the
+ * weights and retains the callbacks.  nstreams >= 1 is the minimum number of
+ * planes kept in flight; the plan may batch more (see DESIGN). */
+struct fmvd_plan_cuda_16 *fmvd_initialize_cuda_16(
+    int dataH, int dataW, float const *const *h_Weights, float **h_Kernel, int kernelH,
+    int kernelW, fmvd_psf_type iteration_type, int nViews, int nstreams,
+    datasource_t_16 get_next_plane, datasink_t_16 return_next_plane);
+struct fmvd_plan_cuda_8 *fmvd_initialize_cuda_8(
+    int dataH, int dataW, float const *const *h_Weights, float **h_Kernel, int kernelH,
+    int kernelW, fmvd_psf_type iteration_type, int nViews, int nstreams,
+    datasource_t_8 get_next_plane, datasink_t_8 return_next_plane);
+
+/* fmvd_deconvolve.impl.h:51-52 / fmvd_deconvolve.impl.cpp:205-300: pull planes
+ * until the datasource returns 0, deconvolve each with `iterations` RL
+ * iterations, push every result plane to the datasink in z order. */
+void fmvd_deconvolve_planes_cuda_16(const struct fmvd_plan_cuda_16 *plan, int iterations);
+void fmvd_deconvolve_planes_cuda_8(const struct fmvd_plan_cuda_8 *plan, int iterations);
+
+/* fmvd_deconvolve.impl.h:54-55 / fmvd_deconvolve.impl.cpp:302-336 */
+void fmvd_destroy_cuda_16(struct fmvd_plan_cuda_16 *plan);
+void fmvd_destroy_cuda_8(struct fmvd_plan_cuda_8 *plan);
+
+/* fmvd_deconvolve.impl.h:57-66 / fmvd_deconvolve_hdd.impl.cpp:36-76: raw
+ * planes read from dataFiles[v] (dataD planes of dataW*dataH samples each),
+ * results written to resultFile in z order. */
+void fmvd_deconvolve_files_cuda_16(FILE **dataFiles, FILE *resultFile, int dataW, int dataH,
+                                   int dataD, float **h_Weights, float **h_Kernel, int kernelH,
+                                   int kernelW, fmvd_psf_type iteration_type, int nViews,
+                                   int iterations);
+void fmvd_deconvolve_files_cuda_8(FILE **dataFiles, FILE *resultFile, int dataW, int dataH,
+                                  int dataD, float **h_Weights, float **h_Kernel, int kernelH,
+                                  int kernelW, fmvd_psf_type iteration_type, int nViews,
+                                  int iterations);
+
+/* ------------------------------------------------------------------------
+ * Extensions with no counterpart in the reference.
+ * ---------------------------------------------------------------------- */
+
+/* Receives the FP32 estimate of each plane (dataW*dataH floats, before the
+ * truncation to samples) just before the datasink sees that plane.  For
+ * testing and for callers that want the float result. */
+typedef void (*fmvd_estimate_sink_t)(const float *estimate, int plane, void *user);
+int fmvd_set_estimate_sink_cuda_16(struct fmvd_plan_cuda_16 *plan, fmvd_estimate_sink_t sink,
+                                   void *user);
+int fmvd_set_estimate_sink_cuda_8(struct fmvd_plan_cuda_8 *plan, fmvd_estimate_sink_t sink,
+                                  void *user);
+
+/* Maximum number of planes processed together (0 = automatic, from a device
+ * memory budget).  Returns the value in effect. */
+int fmvd_set_batch_planes_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes);
+int fmvd_set_batch_planes_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
+
+/* GPU-resident volume deconvolution: d_views[v] (host array of nViews device
+ * pointers) each point to nPlanes planes of dataW*dataH samples in device
+ * memory.  Results go to d_out (nPlanes planes of samples) and/or d_estimate
+ * (nPlanes float planes); either may be NULL.  d_init (nPlanes float planes,
+ * may be NULL) replaces the constant start estimate 10*nViews.  All work is
+ * enqueued on `stream` (a hipStream_t, NULL = the null stream) and the call
+ * returns without synchronising.  Returns 0 on success. */
+int fmvd_deconvolve_volume_device_cuda_16(const struct fmvd_plan_cuda_16 *plan,
+                                          const unsigned short *const *d_views, int nPlanes,
+                                          int iterations, const float *d_init,
+                                          unsigned short *d_out, float *d_estimate, void *stream);
+int fmvd_deconvolve_volume_device_cuda_8(const struct fmvd_plan_cuda_8 *plan,
+                                         const unsigned char *const *d_views, int nPlanes,
+                                         int iterations, const float *d_init, unsigned char *d_out,
+                                         float *d_estimate, void *stream);
+
+/* Plan geometry: fills dims[0..5] = dataH, dataW, fftH, fftW, nViews, batch planes. */
+int fmvd_plan_info_cuda_16(const struct fmvd_plan_cuda_16 *plan, int *dims);
+int fmvd_plan_info_cuda_8(const struct fmvd_plan_cuda_8 *plan, int *dims);
+
+/* FFT extent rule of the reference (fmvd_deconvolve.cpp:28-51) and whether
+ * the device kernels support the resulting extents. */
+int fmvd_snap_transform_size(int n);
+int fmvd_fft_extent_supported(int fftH, int fftW);
+
+/* Per-kernel GPU timing with HIP events recorded around each launch on the
+ * launch stream (off by default).  kind: 0 column pass, 1 ratio row pass,
+ * 2 update row pass, 3 ingest, 4 egress, 5 transform.  fmvd_kernel_timing_read
+ * synchronises on the recorded events and returns the number of launches and
+ * their summed duration in milliseconds. */
+void fmvd_kernel_timing_enable(int on);
+void fmvd_kernel_timing_reset(void);
+int fmvd_kernel_timing_read(int kind, double *total_ms, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMVD_DECONVOLVE_H */
