@@ -91,6 +91,15 @@ def lib() -> C.CDLL:
         "fmvd_transform_device_16": (i, [vp, i, i, i, vp, i, i, i, C.POINTER(f), vp]),
         "fmvd_transform_device_8": (i, [vp, i, i, i, vp, i, i, i, C.POINTER(f), vp]),
         "fmvd_transform_mask_device": (i, [i, i, i, i, i, i, C.POINTER(f), i, f, vp, vp]),
+        "fmvd_device_malloc": (vp, [C.c_size_t]),
+        "fmvd_device_free": (None, [vp]),
+        "fmvd_memcpy": (i, [vp, vp, C.c_size_t, i]),
+        "fmvd_memset_device": (i, [vp, i, C.c_size_t]),
+        "fmvd_device_synchronize": (i, []),
+        "fmvd_stream_create": (vp, []),
+        "fmvd_stream_destroy": (i, [vp]),
+        "fmvd_stream_synchronize": (i, [vp]),
+        "fmvd_device_memory": (i, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -311,3 +320,90 @@ class KernelTiming:
         ms, n = C.c_double(), C.c_longlong()
         lib().fmvd_kernel_timing_read(TIMING_KINDS[kind], C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+
+class DeviceArray:
+    """A device buffer owned by the library's HIP runtime (fmvd_device.h).
+
+    Callers that also load another HIP runtime in the same process (a torch
+    wheel bundles its own) cannot share device pointers with this library,
+    so the tests and the benchmark allocate through here."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        take_errors()
+        self.ptr = lib().fmvd_device_malloc(self.nbytes) or 0
+        check()
+        if not self.ptr:
+            raise FmvdError("device allocation failed")
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(arr)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        lib().fmvd_memcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes, 1)
+        check()
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        lib().fmvd_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2)
+        check()
+        return out
+
+    def fill_zero(self) -> None:
+        lib().fmvd_memset_device(C.c_void_p(self.ptr), 0, self.nbytes)
+        check()
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().fmvd_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream of the library's runtime."""
+
+    def __init__(self):
+        self.handle = lib().fmvd_stream_create() or 0
+        check()
+
+    def synchronize(self) -> None:
+        lib().fmvd_stream_synchronize(C.c_void_p(self.handle))
+        check()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().fmvd_stream_destroy(C.c_void_p(self.handle))
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_synchronize() -> None:
+    lib().fmvd_device_synchronize()
+    check()
+
+
+def device_memory():
+    f, t = C.c_size_t(), C.c_size_t()
+    lib().fmvd_device_memory(C.byref(f), C.byref(t))
+    check()
+    return f.value, t.value

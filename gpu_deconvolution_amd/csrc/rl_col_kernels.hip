@@ -25,15 +25,24 @@ __device__ __forceinline__ unsigned xcd_remap_col(unsigned bid, unsigned nblocks
     return x * q + (x < rem ? x : rem) + k;
 }
 
+constexpr int pow2_floor(int x)
+{
+    int p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+
 template <int H> struct ColCfg {
     using PL = Plan<H>;
     static constexpr int T = PL::T;
     static constexpr int P = PL::P;
     static constexpr int L = lds_len<H>() + 1;
-    // sequences per workgroup: aim for 256 threads, keep LDS <= ~64 KB
+    // sequences per workgroup (a power of two, so it divides the row count):
+    // aim for 256 threads, keep LDS <= ~64 KB
     static constexpr int SQ0 = (256 / T) < 1 ? 1 : (256 / T);
     static constexpr int SQ1 = SQ0 > 16 ? 16 : SQ0;
-    static constexpr int SQ = (SQ1 * L * 8 > 65536 && SQ1 > 1) ? (65536 / (L * 8) < 1 ? 1 : 65536 / (L * 8)) : SQ1;
+    static constexpr int SQ2 = 65536 / (L * 8) < 1 ? 1 : 65536 / (L * 8);
+    static constexpr int SQ = pow2_floor(SQ1 < SQ2 ? SQ1 : SQ2);
     static constexpr int THREADS = SQ * T;
     static constexpr int RF = first_radix<PL, -1>();
     static constexpr int RL = last_radix<PL, -1>();

@@ -1,5 +1,5 @@
-"""Helpers for the GPU parity tests: device buffers via torch (plumbing only)
-and the comparison rules against the oracle."""
+"""Helpers for the GPU parity tests: device buffers from the library's own
+HIP runtime (fmvd_device.h) and the comparison rules against the oracle."""
 import numpy as np
 
 from conftest import relL2
@@ -11,33 +11,18 @@ RL_TOL = 1e-4
 BOUNDARY = 1e-2
 
 
-def torch_cuda():
-    import torch
-    assert torch.cuda.is_available(), "GPU test run without a GPU"
-    return torch
-
-
 def to_device(arr: np.ndarray):
-    torch = torch_cuda()
-    a = np.ascontiguousarray(arr)
-    if a.dtype == np.uint16:
-        t = torch.from_numpy(a.view(np.int16)).cuda()
-    else:
-        t = torch.from_numpy(a).cuda()
-    return t
+    import gpu_deconvolution_amd as fm
+    return fm.DeviceArray.from_host(arr)
 
 
-def from_device(t, dtype):
-    a = t.cpu().numpy()
-    if dtype == np.uint16:
-        return a.view(np.uint16)
-    return a.astype(dtype, copy=False)
+def from_device(d, dtype=None):
+    return d.download()
 
 
 def empty_device(shape, dtype):
-    torch = torch_cuda()
-    tdt = {np.uint16: torch.int16, np.uint8: torch.uint8, np.float32: torch.float32}[dtype]
-    return torch.empty(tuple(shape), dtype=tdt, device="cuda")
+    import gpu_deconvolution_amd as fm
+    return fm.DeviceArray(shape, dtype)
 
 
 def check_rl(out_gpu, est_gpu, out_orc, est_orc, tol=RL_TOL, bits=16):

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from gpu_util import empty_device, from_device, to_device, torch_cuda
+from gpu_util import empty_device, from_device, to_device
 
 pytestmark = pytest.mark.gpu
 
@@ -45,13 +45,12 @@ def test_transform_device_bit_exact(fm, orc, bits, name):
     src = volume(shape, bits, seed=len(name) + bits)
     M = MATS[name](shape)
     print(f"\n[transform] bits={bits} mat={name}", flush=True)
-    torch = torch_cuda()
     dt = np.uint16 if bits == 16 else np.uint8
     dsrc = to_device(src)
     ddst = empty_device(tshape, dt)
-    fm.transform_device(dsrc.data_ptr(), shape, ddst.data_ptr(), tshape, M, bits=bits,
-                        stream=torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
+    fm.transform_device(dsrc.ptr, shape, ddst.ptr, tshape, M, bits=bits,
+                        stream=0)
+    fm.device_synchronize()
     got = from_device(ddst, dt)
     want = orc.transform(src.astype(np.uint16), M, tshape, bits=bits).astype(dt)
     assert np.array_equal(got, want), int(np.count_nonzero(got != want))
@@ -89,11 +88,10 @@ def test_transform_file_api_8bit_many_chunks(fm, orc, tmp_path):
 
 
 def test_mask_device(fm, orc):
-    torch = torch_cuda()
     shape, tshape = (40, 50, 60), (30, 52, 64)
     M = affine(3, shape)
     d = empty_device(tshape[1:], np.float32)
-    fm.mask_device(shape, tshape, M, 10, 1.5, d.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
+    fm.mask_device(shape, tshape, M, 10, 1.5, d.ptr, stream=0)
+    fm.device_synchronize()
     got = from_device(d, np.float32)
     assert np.abs(got - orc.mask(shape, tshape, M, 10, 1.5)).max() <= 2 * np.finfo(np.float32).eps
