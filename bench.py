@@ -1,0 +1,214 @@
+"""Benchmark: affine transform + plane-wise multi-view RL at real size, one MI355X.
+
+Workload (the metric's real-size configuration): four uint16 views of
+1024 x 1024 x 512 (20k Gaussian beads on background 10, per-view anisotropic
+Gaussian PSF, Poisson noise), each warped by a random affine (rotation
++-15 deg about each axis, scale 0.9-1.1, translation +-20 voxels).  One step
+= resample the four views back with the trilinear transform and run 30 RL
+iterations over all 512 planes with 41 x 41 PSFs (FFT extent 1536 x 1536),
+weights = the transform's border masks.  Inputs are resident in HBM when the
+timed region starts; data is synthetic and generated on the GPU.
+
+Prints one JSON line.  value = view-voxel RL iterations per second over the
+whole step (V * W * H * D * iterations / step time).  roofline: the
+dominant kernel's algorithmic bytes per launch (from the per-plane pass
+table: column pass 16 S, ratio pass 16 S + 2 P, update pass 16 S + 8 P bytes
+with S = fftH (fftW/2 + 1), P = fftH fftW) over its mean launch time measured
+with HIP events on its launch stream.  cpu_baseline: the FP64 oracle on a
+bounded z-subset of the same registered views, timed on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "view-voxel RL iters/s + time to 1e-3 rel-L2 vs FP64 RL: learned vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--planes", type=int, default=512, help="volume depth (default: the workload's 512)")
+    p.add_argument("--iterations", type=int, default=30)
+    p.add_argument("--cpu-planes", type=int, default=16)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import gpu_deconvolution_amd as fm
+    from gpu_deconvolution_amd import synth
+
+    Z, H, W, V, K, ITERS = args.planes, 1024, 1024, 4, 41, args.iterations
+    seed = 1001
+    shape = (Z, H, W)
+    fm.lib().setCudaDevice(0)
+    fm.check()
+    name = C.create_string_buffer(256)
+    fm.lib().getCudaDeviceName(0, name)
+
+    t0 = time.time()
+    raw = synth.gpu_views(shape, V, beads=20000, seed=seed)
+    rng = np.random.default_rng(seed + 7)
+    A = [synth.random_affine(rng, shape) for _ in range(V)]
+    acquired = []
+    for v in range(V):
+        d = fm.DeviceArray(shape, np.uint16)
+        fm.transform_device(raw[v].ptr, shape, d.ptr, shape, synth.invert_affine(A[v]))
+        acquired.append(d)
+    fm.device_synchronize()
+    for r in raw:
+        r.free()
+    kernels = [synth.gaussian_kernel_2d(*synth.view_sigma((1.5, 1.5, 4.0), v)[:2], K) for v in range(V)]
+    weights = []
+    mask = fm.DeviceArray((H, W), np.float32)
+    for v in range(V):
+        fm.mask_device(shape, shape, A[v], 32, 1.0, mask.ptr)
+        weights.append(mask.download())
+    registered = [fm.DeviceArray(shape, np.uint16) for _ in range(V)]
+    out = fm.DeviceArray(shape, np.uint16)
+    log(f"[bench] data ready in {time.time() - t0:.1f}s on {name.value.decode()}")
+
+    t0 = time.time()
+    dec = fm.Deconvolver(H, W, weights, [k.copy() for k in kernels], psf_type=0, bits=16, batch_planes=Z)
+    info = dec.info()
+    log(f"[bench] plan {info} in {time.time() - t0:.1f}s")
+    stream = fm.Stream()
+
+    def step():
+        for v in range(V):
+            fm.transform_device(acquired[v].ptr, shape, registered[v].ptr, shape, A[v], stream=stream.handle)
+        dec.run_device([r.ptr for r in registered], Z, ITERS, out_ptr=out.ptr, stream=stream.handle)
+
+    for i in range(args.warmup):
+        t = time.time()
+        step()
+        stream.synchronize()
+        log(f"[bench] warmup {i}: {time.time() - t:.3f}s")
+
+    fm.device_synchronize()
+    with fm.KernelTiming() as kt:
+        t = time.perf_counter()
+        for i in range(args.steps):
+            step()
+        stream.synchronize()
+        elapsed = time.perf_counter() - t
+        timing = {k: kt.read(k) for k in ("column", "ratio", "update", "ingest", "egress", "transform")}
+    ms_per_step = 1000.0 * elapsed / args.steps
+    vvi = V * W * H * Z * ITERS
+    value = vvi / (elapsed / args.steps)
+    log(f"[bench] {args.steps} steps in {elapsed:.3f}s -> {ms_per_step:.1f} ms/step, {value:.4g} view-voxel it/s")
+
+    fh, fw = info["fft_h"], info["fft_w"]
+    S, P = fh * (fw // 2 + 1), fh * fw
+    per_plane = {"column": 16 * S, "ratio": 16 * S + 2 * P, "update": 16 * S + 8 * P}
+    planes_per_launch = min(Z, info["batch"])
+    kstats = {}
+    for k, b in per_plane.items():
+        ms, n = timing[k]
+        if n:
+            avg = ms / n / 1000.0
+            kstats[k] = dict(total_ms=ms, launches=n, avg_ms=1000 * avg,
+                             achieved_gbs=b * planes_per_launch / avg / 1e9)
+    dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
+    B_RL = 64 * S + 10 * P
+    rl_ms = sum(kstats[k]["total_ms"] for k in kstats)
+    iter_gbs = B_RL * V * ITERS * Z * args.steps / (rl_ms / 1000.0) / 1e9
+    tr_ms, tr_n = timing["transform"]
+    tr_gbs = (2 * Z * H * W * 2) * tr_n / (tr_ms / 1000.0) / 1e9 if tr_n else None
+    for k, s in kstats.items():
+        log(f"[bench] {k:7s} {s['launches']:5d} launches avg {s['avg_ms']:.3f} ms  {s['achieved_gbs']:.0f} GB/s")
+    log(f"[bench] RL chain (B_RL = 64S+10P = {B_RL / 1e6:.2f} MB/view-iter-plane): {iter_gbs:.0f} GB/s; "
+        f"transform {tr_gbs and round(tr_gbs)} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        zc = min(args.cpu_planes, Z)
+        z0 = Z // 2 - zc // 2
+        vols = [registered[v].download()[z0:z0 + zc] for v in range(V)]
+        data = np.ascontiguousarray(np.stack(vols, axis=1))  # (zc, V, H, W)
+        del vols
+        kern = np.stack(kernels)
+        wts = np.stack(weights)
+        nt = cpu_threads()
+        t = time.perf_counter()
+        oracle.rl_planes(data, wts, kern.copy(), 0, 0, threads=nt)
+        t_setup = time.perf_counter() - t
+        t = time.perf_counter()
+        oracle.rl_planes(data, wts, kern.copy(), 0, 1, threads=nt)
+        t_one = time.perf_counter() - t
+        t_iter = max(t_one - t_setup, 1e-9)
+        cpu = {"value": V * W * H * zc * 1 / t_iter, "unit": "view-voxel RL iterations/s", "cores": nt,
+               "kind": "port",
+               "sample": f"FP64 oracle, {zc} planes x {V} views x 1 iteration of the same registered volume "
+                         f"(FFT {fh}x{fw}), OpenMP over planes; plan setup ({t_setup:.1f}s) subtracted"}
+        log(f"[bench] cpu baseline {cpu['value']:.4g} view-voxel it/s on {nt} threads ({t_iter:.1f}s)")
+
+    dk = kstats[dom]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "view-voxel RL iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (GPU-generated beads, Poisson noise, random affines)",
+        "config": {
+            "workload": "affine transform + plain multi-view RL, 4 views uint16 1024x1024x512, "
+                        "41x41 PSFs, 30 iterations, FFT 1536x1536",
+            "views": V, "shape_zyx": [Z, H, W], "kernel": K, "iterations": ITERS, "fft": [fh, fw],
+            "planes_per_launch": planes_per_launch, "device": name.value.decode(),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": dk["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": dk["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "rl_iteration_roofline": {"achieved": iter_gbs, "frac": iter_gbs / HBM_PEAK_GBS, "unit": "GB/s",
+                                  "bytes_per_view_iter_plane": B_RL},
+        "kernels": kstats,
+        "transform_gbs": tr_gbs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

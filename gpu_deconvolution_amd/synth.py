@@ -119,3 +119,59 @@ def invert_affine(m: np.ndarray) -> np.ndarray:
     M = np.eye(4)
     M[:3] = np.asarray(m, np.float64).reshape(3, 4)
     return np.linalg.inv(M)[:3].astype(np.float32)
+
+
+# ----------------------------------------------------------------- GPU side
+_synth = None
+
+
+def synth_lib():
+    """libfmvd_synth.so: GPU generator for large workloads (same HIP runtime
+    as the reconstruction library)."""
+    global _synth
+    if _synth is None:
+        import ctypes as C
+        import os
+        from . import lib
+        lib()
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfmvd_synth.so")
+        L = C.CDLL(path)
+        vp, i = C.c_void_p, C.c_int
+        L.fmvd_synth_truth.argtypes = [vp, i, i, i, vp, i, C.c_float, vp]
+        L.fmvd_synth_blur.argtypes = [vp, vp, i, i, i, i, C.c_float, vp]
+        L.fmvd_synth_poisson.argtypes = [vp, vp, C.c_long, C.c_ulonglong, i, vp]
+        _synth = L
+    return _synth
+
+
+def gpu_views(shape, views, beads, seed, sigma=(1.5, 1.5, 4.0), bits=16):
+    """Ground truth with `beads` beads, blurred per view (base PSF rotated
+    v*90 degrees about y) and Poisson-noised, on the device.  Returns a list
+    of DeviceArrays (Z, H, W) of samples, one per view."""
+    import ctypes as C
+    from . import DeviceArray, check, device_synchronize
+    L = synth_lib()
+    Z, H, W = shape
+    rng = np.random.default_rng(seed)
+    b = np.stack([rng.uniform(0, Z, beads), rng.uniform(0, H, beads), rng.uniform(0, W, beads),
+                  rng.uniform(1.0, 3.0, beads), rng.uniform(100.0, 1000.0, beads)], 1).astype(np.float32)
+    truth = DeviceArray(shape, np.float32)
+    t1 = DeviceArray(shape, np.float32)
+    t2 = DeviceArray(shape, np.float32)
+    L.fmvd_synth_truth(C.c_void_p(truth.ptr), Z, H, W, b.ctypes.data_as(C.c_void_p), beads, 10.0, None)
+    check()
+    out = []
+    dt = np.uint16 if bits == 16 else np.uint8
+    for v in range(views):
+        sx, sy, sz = view_sigma(sigma, v)
+        L.fmvd_synth_blur(C.c_void_p(truth.ptr), C.c_void_p(t1.ptr), Z, H, W, 0, sx, None)
+        L.fmvd_synth_blur(C.c_void_p(t1.ptr), C.c_void_p(t2.ptr), Z, H, W, 1, sy, None)
+        L.fmvd_synth_blur(C.c_void_p(t2.ptr), C.c_void_p(t1.ptr), Z, H, W, 2, sz, None)
+        d = DeviceArray(shape, dt)
+        L.fmvd_synth_poisson(C.c_void_p(t1.ptr), C.c_void_p(d.ptr), Z * H * W, (seed << 8) + v, bits, None)
+        check()
+        out.append(d)
+    device_synchronize()
+    for x in (truth, t1, t2):
+        x.free()
+    return out
