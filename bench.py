@@ -100,6 +100,10 @@ def main():
     dec = fm.Deconvolver(H, W, weights, [k.copy() for k in kernels], psf_type=0, bits=16, batch_planes=Z)
     info = dec.info()
     log(f"[bench] plan {info} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    tuned = dec.autotune(planes=64)
+    log(f"[bench] autotune in {time.time() - t0:.1f}s: " +
+        ", ".join(f"{k}=v{c} {['%.3f' % x if x is not None else '-' for x in ts]}" for k, (c, ts) in tuned.items()))
     stream = fm.Stream()
 
     def step():
@@ -191,6 +195,7 @@ def main():
                         "41x41 PSFs, 30 iterations, FFT 1536x1536",
             "views": V, "shape_zyx": [Z, H, W], "kernel": K, "iterations": ITERS, "fft": [fh, fw],
             "planes_per_launch": planes_per_launch, "device": name.value.decode(),
+            "fft_plan_variants": {k: c for k, (c, _) in tuned.items()},
         },
         "roofline": {
             "bound": "hbm",

@@ -91,6 +91,10 @@ def lib() -> C.CDLL:
         "fmvd_transform_device_16": (i, [vp, i, i, i, vp, i, i, i, C.POINTER(f), vp]),
         "fmvd_transform_device_8": (i, [vp, i, i, i, vp, i, i, i, C.POINTER(f), vp]),
         "fmvd_transform_mask_device": (i, [i, i, i, i, i, i, C.POINTER(f), i, f, vp, vp]),
+        "fmvd_autotune_cuda_16": (i, [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+        "fmvd_autotune_cuda_8": (i, [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+        "fmvd_set_variants_cuda_16": (i, [vp, C.POINTER(C.c_int)]),
+        "fmvd_set_variants_cuda_8": (i, [vp, C.POINTER(C.c_int)]),
         "fmvd_device_malloc": (vp, [C.c_size_t]),
         "fmvd_device_free": (None, [vp]),
         "fmvd_memcpy": (i, [vp, vp, C.c_size_t, i]),
@@ -215,6 +219,29 @@ class Deconvolver:
 
     def set_batch_planes(self, n: int) -> int:
         return self._set_batch(self.plan, int(n))
+
+    KINDS = ("column", "ratio", "update", "ingest", "egress")
+
+    def autotune(self, planes: int = 32) -> dict:
+        """Times every compiled FFT-plan variant per kernel kind and keeps the
+        fastest.  Returns {kind: (chosen, [ms per variant or None])}."""
+        L = lib()
+        fn = L.fmvd_autotune_cuda_16 if self.bits == 16 else L.fmvd_autotune_cuda_8
+        chosen = (C.c_int * 5)()
+        times = (C.c_float * 15)()
+        rc = fn(self.plan, int(planes), chosen, times)
+        check()
+        if rc != 0:
+            raise FmvdError("autotune failed")
+        return {k: (chosen[i], [times[3 * i + v] if times[3 * i + v] >= 0 else None for v in range(3)])
+                for i, k in enumerate(self.KINDS)}
+
+    def set_variants(self, variants) -> None:
+        L = lib()
+        fn = L.fmvd_set_variants_cuda_16 if self.bits == 16 else L.fmvd_set_variants_cuda_8
+        arr = (C.c_int * 5)(*[int(x) for x in variants])
+        if fn(self.plan, arr) != 0:
+            raise FmvdError(f"invalid variants {list(variants)}")
 
     def run(self, planes: Sequence[Sequence[np.ndarray]], iterations: int, want_estimate: bool = False):
         """planes: sequence over z of per-view (dh, dw) sample arrays.

@@ -278,6 +278,41 @@ template <> struct Plan<3072> : PlanDef<3072, 64, RadixList<12, 16, 16>> {};
 template <> struct Plan<3584> : PlanDef<3584, 64, RadixList<7, 8, 8, 8>> {};
 template <> struct Plan<4096> : PlanDef<4096, 256, RadixList<16, 16, 16>> {};
 
+// Alternative plans for the extents of the benchmark workloads, selected at
+// run time per kernel kind (fewer registers per thread at the price of more
+// LDS exchanges, or the reverse).  PlanV<N, 0> is Plan<N>.
+constexpr int kPlanVariants = 3;
+template <int N, int V> struct PlanV : Plan<N> {
+    static constexpr bool distinct = (V == 0);
+};
+#define FMVD_PLAN_VARIANT(n, v, t, ...)                                          \
+    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>> {     \
+        static constexpr bool distinct = true;                                   \
+    };
+FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
+FMVD_PLAN_VARIANT(512, 1, 64, 8, 8, 8)
+FMVD_PLAN_VARIANT(768, 1, 64, 12, 4, 4, 4)
+FMVD_PLAN_VARIANT(768, 2, 64, 3, 4, 4, 4, 4)
+FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
+FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
+FMVD_PLAN_VARIANT(1280, 1, 32, 20, 8, 8)
+FMVD_PLAN_VARIANT(1280, 2, 64, 5, 4, 4, 4, 4)
+FMVD_PLAN_VARIANT(1536, 1, 64, 3, 8, 8, 8)
+FMVD_PLAN_VARIANT(1536, 2, 64, 12, 4, 4, 8)
+FMVD_PLAN_VARIANT(2560, 1, 64, 10, 4, 8, 8)
+#undef FMVD_PLAN_VARIANT
+
+// A plan is valid when its radices multiply to N and each divides P.
+template <typename PL> constexpr bool plan_valid()
+{
+    int prod = 1;
+    for (int i = 0; i < PL::NP; ++i) {
+        if (PL::P % PL::R::at(i) != 0) return false;
+        prod *= PL::R::at(i);
+    }
+    return prod == PL::N && PL::N % PL::T == 0;
+}
+
 // Compile-time product of radices [0, i)
 template <typename PL> constexpr int span_before(int i)
 {
@@ -357,10 +392,27 @@ __device__ __forceinline__ void lds_load_in(const float2* lds, float2* v, int t)
         for (int r = 0; r < R; ++r) v[b * R + r] = lds[lds_pad(in_pos<PL, R>(t, b, r))];
 }
 
+// Synchronisation of the threads that own one sequence.  When a sequence's T
+// threads lie inside one wavefront (T <= 64, sequences aligned to T), LDS
+// traffic between them only needs wavefront ordering: a wavefront's LDS
+// instructions execute in issue order, so a compiler barrier plus wavefront
+// fences suffice and other wavefronts of the workgroup keep running.
+// Larger sequences need the workgroup barrier.
+template <typename PL> __device__ __forceinline__ void seq_sync()
+{
+    if constexpr (PL::T <= 64 && (64 % PL::T) == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Runs passes [I, NP) of a transform whose pass I-1 has already been computed
 // in registers (or I == 0 with the first pass's inputs in registers).  The
-// exchange between passes goes through `lds` (this sequence's slice) with
-// workgroup barriers; every thread of the workgroup must call this.
+// exchange between passes goes through `lds` (this sequence's slice); all T
+// threads of the sequence must call this.
 template <typename PL, int DIR, int I>
 __device__ __forceinline__ void run_passes_from(float2* v, float2* lds, int t, const float2* __restrict__ tw)
 {
@@ -371,9 +423,9 @@ __device__ __forceinline__ void run_passes_from(float2* v, float2* lds, int t, c
             constexpr int RP = radix_of<PL, DIR>(I - 1);
             constexpr int NSP = ns_of<PL, DIR>(I - 1);
             lds_store_out<PL, RP, NSP>(lds, v, t);
-            __syncthreads();
+            seq_sync<PL>();
             lds_load_in<PL, R>(lds, v, t);
-            __syncthreads();
+            seq_sync<PL>();
         }
         pass_compute<PL, R, NS, DIR>(v, t, tw);
         run_passes_from<PL, DIR, I + 1>(v, lds, t, tw);
@@ -383,6 +435,8 @@ __device__ __forceinline__ void run_passes_from(float2* v, float2* lds, int t, c
 // Full transform, first-pass inputs already in registers at in_pos of the
 // first radix; on exit the outputs are in registers at in_pos of the last
 // radix of this direction (natural-order position p = j + r*N/R_last).
+// The caller must have ordered any earlier LDS reads of this sequence's slice
+// before the call (seq_sync or a workgroup barrier).
 template <typename PL, int DIR>
 __device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, const float2* __restrict__ tw)
 {

@@ -32,8 +32,10 @@ constexpr int pow2_floor(int x)
     return p;
 }
 
-template <int H> struct ColCfg {
-    using PL = Plan<H>;
+template <typename PL_> struct ColCfg {
+    using PL = PL_;
+    static_assert(plan_valid<PL>(), "invalid FFT plan");
+    static constexpr int H = PL::N;
     static constexpr int T = PL::T;
     static constexpr int P = PL::P;
     static constexpr int L = lds_len<H>() + 1;
@@ -50,13 +52,13 @@ template <int H> struct ColCfg {
     static_assert(RF == last_radix<PL, +1>(), "inverse exit layout must match forward entry");
 };
 
-template <int H>
-__global__ __launch_bounds__(ColCfg<H>::THREADS) void col_conv_kernel(
-    ColGeom g, const float2* __restrict__ Xin, float2* __restrict__ Xout, long xstride,
-    const float2* __restrict__ K, int nplanes)
+template <typename PL>
+__global__ __launch_bounds__(ColCfg<PL>::THREADS) void col_conv_kernel(ColGeom g, const float2* __restrict__ Xin,
+                                                                       float2* __restrict__ Xout, long xstride,
+                                                                       const float2* __restrict__ K, int nplanes)
 {
-    using C = ColCfg<H>;
-    using PL = typename C::PL;
+    using C = ColCfg<PL>;
+    constexpr int H = C::H;
     __shared__ float2 lds[C::SQ * C::L];
     const unsigned lid = xcd_remap_col(blockIdx.x, gridDim.x);
     // plane-fastest: consecutive logical blocks reuse the same kernel rows
@@ -71,10 +73,12 @@ __global__ __launch_bounds__(ColCfg<H>::THREADS) void col_conv_kernel(
     for (int i = 0; i < C::P; ++i) v[i] = xin[slot_pos<PL, C::RF>(t, i)];
     fft_regs<PL, -1>(v, row, t, g.twH);
 
-    if (grp == 0) {  // block-uniform: this block owns the packed kx = 0 sequence
+    if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
+        if (seq == 0) {
 #pragma unroll
-        for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RL>(t, i))] = v[i];
-        __syncthreads();
+            for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RL>(t, i))] = v[i];
+        }
+        seq_sync<PL>();
         if (seq == 0) {
             const float2* K0 = K;
             const float2* KN = K + (long)g.N * H;
@@ -95,7 +99,7 @@ __global__ __launch_bounds__(ColCfg<H>::THREADS) void col_conv_kernel(
 #pragma unroll
             for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[slot_pos<PL, C::RL>(t, i)]);
         }
-        __syncthreads();
+        seq_sync<PL>();
     } else {
         const float2* Kr = K + (long)kx * H;
 #pragma unroll
@@ -108,7 +112,8 @@ __global__ __launch_bounds__(ColCfg<H>::THREADS) void col_conv_kernel(
     for (int i = 0; i < C::P; ++i) xo[slot_pos<PL, C::RF>(t, i)] = v[i];
 }
 
-#define FMVD_COL_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(3072) X(3584) X(4096)
+#define FMVD_COL_SIZES(X) \
+    X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(3072) X(3584) X(4096)
 
 bool col_size_supported(int H)
 {
@@ -122,18 +127,53 @@ bool col_size_supported(int H)
     }
 }
 
-void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride,
-                     const float2* K, int nplanes, hipStream_t s)
+int col_variants(int H)
+{
+    int n = 0;
+    switch (H) {
+#define FMVD_CASE(nn)                                                                        \
+    case nn:                                                                                 \
+        n = 1 + (PlanV<nn, 1>::distinct ? 1 : 0) + (PlanV<nn, 2>::distinct ? 1 : 0);         \
+        break;
+        FMVD_COL_SIZES(FMVD_CASE)
+#undef FMVD_CASE
+    default:
+        break;
+    }
+    return n;
+}
+
+template <typename PL>
+static void launch_col_plan(const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K,
+                            int nplanes, hipStream_t s)
+{
+    using C = ColCfg<PL>;
+    if (g.N % C::SQ) throw_msg("row count not divisible by column group");
+    dim3 grid((unsigned)(g.N / C::SQ * nplanes));
+    col_conv_kernel<PL><<<grid, C::THREADS, 0, s>>>(g, Xin, Xout, xstride, K, nplanes);
+}
+
+template <int H>
+static void col_variant(int v, const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K,
+                        int nplanes, hipStream_t s)
+{
+    if (v == 1 && PlanV<H, 1>::distinct) {
+        if constexpr (PlanV<H, 1>::distinct) launch_col_plan<PlanV<H, 1>>(g, Xin, Xout, xstride, K, nplanes, s);
+    } else if (v == 2 && PlanV<H, 2>::distinct) {
+        if constexpr (PlanV<H, 2>::distinct) launch_col_plan<PlanV<H, 2>>(g, Xin, Xout, xstride, K, nplanes, s);
+    } else {
+        launch_col_plan<PlanV<H, 0>>(g, Xin, Xout, xstride, K, nplanes, s);
+    }
+}
+
+void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K, int nplanes,
+                     int variant, hipStream_t s)
 {
     switch (g.H) {
-#define FMVD_CASE(n)                                                                       \
-    case n: {                                                                              \
-        if (g.N % ColCfg<n>::SQ) throw_msg("row count not divisible by column group");     \
-        dim3 grid((unsigned)(g.N / ColCfg<n>::SQ * nplanes));                              \
-        col_conv_kernel<n><<<grid, ColCfg<n>::THREADS, 0, s>>>(g, Xin, Xout, xstride, K,   \
-                                                               nplanes);                   \
-        break;                                                                             \
-    }
+#define FMVD_CASE(n)                                                          \
+    case n:                                                                   \
+        col_variant<n>(variant, g, Xin, Xout, xstride, K, nplanes, s);        \
+        break;
         FMVD_COL_SIZES(FMVD_CASE)
 #undef FMVD_CASE
     default:

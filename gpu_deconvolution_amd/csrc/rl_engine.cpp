@@ -125,6 +125,7 @@ template <typename S> struct Plan {
     fmvd_estimate_sink_t esink = nullptr;
     void* euser = nullptr;
     int batchCap = 0;
+    Variants var;
 
     DeviceBuffer twN, twW, twH;
     std::vector<DeviceBuffer> K, Kh, Wn;
@@ -247,31 +248,99 @@ template <typename S> struct Plan {
         float2* xt = XT.as<float2>();
         {
             TimedScope ts(kTimeIngest, s);
-            if (init) launch_row_init(rg, xe, xs, init, istride, np, s);
+            if (init) launch_row_init(rg, xe, xs, init, istride, np, var.ingest, s);
             else launch_const_init(rg, xe, xs, 10.0f * (float)nViews, np, s);
         }
         for (int it = 0; it < iterations; ++it) {
             for (int v = 0; v < nViews; ++v) {
                 {
                     TimedScope ts(kTimeColumn, s);
-                    launch_col_conv(cg, xe, xt, xs, K[v].as<float2>(), np, s);
+                    launch_col_conv(cg, xe, xt, xs, K[v].as<float2>(), np, var.column, s);
                 }
                 {
                     TimedScope ts(kTimeRatio, s);
-                    launch_row_ratio<S>(rg, xt, xs, views[v], dstride, np, s);
+                    launch_row_ratio<S>(rg, xt, xs, views[v], dstride, np, var.ratio, s);
                 }
                 {
                     TimedScope ts(kTimeColumn, s);
-                    launch_col_conv(cg, xt, xt, xs, Kh[v].as<float2>(), np, s);
+                    launch_col_conv(cg, xt, xt, xs, Kh[v].as<float2>(), np, var.column, s);
                 }
                 {
                     TimedScope ts(kTimeUpdate, s);
-                    launch_row_update(rg, xe, xt, xs, Wn[v].as<float>(), np, s);
+                    launch_row_update(rg, xe, xt, xs, Wn[v].as<float>(), np, var.update, s);
                 }
             }
         }
         TimedScope ts(kTimeEgress, s);
-        launch_row_final<S>(rg, xe, xs, out, est, ostride, np, s);
+        launch_row_final<S>(rg, xe, xs, out, est, ostride, np, var.egress, s);
+    }
+
+    // Autotuner: times every compiled FFT-plan variant of each kernel kind on
+    // this plan's own buffers (planes planes, synthetic content) and keeps the
+    // fastest.  times[kind*3 + variant] receives the mean launch time in ms
+    // (negative where the variant does not exist).
+    void autotune(int planes, float* times)
+    {
+        FMVD_HIP(hipSetDevice(device));
+        planes = std::max(1, std::min(planes > 0 ? planes : 32, batch_planes()));
+        ensure_work(planes);
+        const size_t pl = plane();
+        DeviceBuffer data, out, e0;
+        data.alloc(sizeof(S) * pl * planes);
+        out.alloc(sizeof(S) * pl * planes);
+        e0.alloc(sizeof(float) * pl * planes);
+        hipStream_t s = compute->s;
+        FMVD_HIP(hipMemsetAsync(data.p, 0x3c, data.bytes, s));
+        FMVD_HIP(hipMemsetAsync(e0.p, 0x42, e0.bytes, s));  // ~ 48.6f
+        const RowGeom rg = row_geom();
+        const ColGeom cg = col_geom();
+        const long xs = (long)xplane();
+        float2* xe = XE.as<float2>();
+        float2* xt = XT.as<float2>();
+        launch_const_init(rg, xe, xs, 10.0f * (float)nViews, planes, s);
+        launch_col_conv(cg, xe, xt, xs, K[0].as<float2>(), planes, 0, s);
+        hipEvent_t a, b;
+        FMVD_HIP(hipEventCreate(&a));
+        FMVD_HIP(hipEventCreate(&b));
+        auto time_it = [&](auto&& launch) -> float {
+            launch();  // warm-up
+            FMVD_HIP(hipEventRecord(a, s));
+            const int reps = 3;
+            for (int r = 0; r < reps; ++r) launch();
+            FMVD_HIP(hipEventRecord(b, s));
+            FMVD_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            FMVD_HIP(hipEventElapsedTime(&ms, a, b));
+            return ms / reps;
+        };
+        const int nc = col_variants(fftH), nr = row_variants(N);
+        int* slots[5] = {&var.column, &var.ratio, &var.update, &var.ingest, &var.egress};
+        for (int k = 0; k < 5; ++k) {
+            const int nv = k == 0 ? nc : nr;
+            float best = 0.f;
+            int bestv = 0;
+            for (int v = 0; v < 3; ++v) {
+                float ms = -1.f;
+                if (v < nv) {
+                    switch (k) {
+                    case 0: ms = time_it([&] { launch_col_conv(cg, xt, xt, xs, Kh[0].as<float2>(), planes, v, s); }); break;
+                    case 1: ms = time_it([&] { launch_row_ratio<S>(rg, xt, xs, data.as<S>(), (long)pl, planes, v, s); }); break;
+                    case 2: ms = time_it([&] { launch_row_update(rg, xe, xt, xs, Wn[0].as<float>(), planes, v, s); }); break;
+                    case 3: ms = time_it([&] { launch_row_init(rg, xt, xs, e0.as<float>(), (long)pl, planes, v, s); }); break;
+                    case 4: ms = time_it([&] { launch_row_final<S>(rg, xe, xs, out.as<S>(), nullptr, (long)pl, planes, v, s); }); break;
+                    }
+                    if (v == 0 || ms < best) {
+                        best = ms;
+                        bestv = v;
+                    }
+                }
+                if (times) times[k * 3 + v] = ms;
+            }
+            *slots[k] = bestv;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        FMVD_HIP(hipStreamSynchronize(s));
     }
 
     void run_volume(const S* const* dviews, int nplanes, int iterations, const float* init, S* out,
@@ -416,6 +485,38 @@ template <typename PlanT> int plan_info(const PlanT* p, int* dims)
     return 0;
 }
 
+template <typename PlanT> static int autotune_plan(PlanT* plan, int planes, int* chosen, float* times_ms)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        plan->impl.autotune(planes, times_ms);
+        if (chosen) {
+            const fmvd::Variants& v = plan->impl.var;
+            chosen[0] = v.column;
+            chosen[1] = v.ratio;
+            chosen[2] = v.update;
+            chosen[3] = v.ingest;
+            chosen[4] = v.egress;
+        }
+        return 0;
+    })
+}
+
+template <typename PlanT> static int set_variants(PlanT* plan, const int* v)
+{
+    if (!plan || !v) return -1;
+    const int nc = fmvd::col_variants(plan->impl.fftH), nr = fmvd::row_variants(plan->impl.N);
+    for (int k = 0; k < 5; ++k)
+        if (v[k] < 0 || v[k] >= (k == 0 ? nc : nr)) return -1;
+    fmvd::Variants& x = plan->impl.var;
+    x.column = v[0];
+    x.ratio = v[1];
+    x.update = v[2];
+    x.ingest = v[3];
+    x.egress = v[4];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -525,6 +626,17 @@ int fmvd_deconvolve_volume_device_cuda_8(const struct fmvd_plan_cuda_8* plan,
         return 0;
     })
 }
+
+int fmvd_autotune_cuda_16(struct fmvd_plan_cuda_16* plan, int planes, int* chosen, float* times_ms)
+{
+    return autotune_plan(plan, planes, chosen, times_ms);
+}
+int fmvd_autotune_cuda_8(struct fmvd_plan_cuda_8* plan, int planes, int* chosen, float* times_ms)
+{
+    return autotune_plan(plan, planes, chosen, times_ms);
+}
+int fmvd_set_variants_cuda_16(struct fmvd_plan_cuda_16* plan, const int* variants) { return set_variants(plan, variants); }
+int fmvd_set_variants_cuda_8(struct fmvd_plan_cuda_8* plan, const int* variants) { return set_variants(plan, variants); }
 
 int fmvd_plan_info_cuda_16(const struct fmvd_plan_cuda_16* plan, int* dims) { return plan_info(plan, dims); }
 int fmvd_plan_info_cuda_8(const struct fmvd_plan_cuda_8* plan, int* dims) { return plan_info(plan, dims); }

@@ -35,25 +35,34 @@ struct ColGeom {
 // data: first plane of this view; dstride: elements between planes.
 template <typename SAMPLE>
 void launch_row_ratio(const RowGeom& g, float2* X, long xstride, const SAMPLE* data, long dstride,
-                      int nplanes, hipStream_t s);
+                      int nplanes, int variant, hipStream_t s);
 // Pass D: E = C2R(XE); C = C2R(XC); E += w*(E*C - E); XE <- R2C(E)/W.
 void launch_row_update(const RowGeom& g, float2* XE, const float2* XC, long xstride,
-                       const float* weights, int nplanes, hipStream_t s);
+                       const float* weights, int nplanes, int variant, hipStream_t s);
 // Egress: E = C2R(XE) on rows < dataH; out = sample(E), est = E (either may be null).
 template <typename SAMPLE>
 void launch_row_final(const RowGeom& g, const float2* XE, long xstride, SAMPLE* out, float* est,
-                      long ostride, int nplanes, hipStream_t s);
+                      long ostride, int nplanes, int variant, hipStream_t s);
 // Warm-start ingest: XE <- R2C(pad(E0))/W, E0 is dataH x dataW per plane.
 void launch_row_init(const RowGeom& g, float2* XE, long xstride, const float* e0, long estride,
-                     int nplanes, hipStream_t s);
+                     int nplanes, int variant, hipStream_t s);
 // Constant start E0 = value everywhere: XE[0][y] = (value, 0), rest 0.
 void launch_const_init(const RowGeom& g, float2* XE, long xstride, float value, int nplanes,
                        hipStream_t s);
 // Passes A / C: Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]) for every kx.
 void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride,
-                     const float2* K, int nplanes, hipStream_t s);
+                     const float2* K, int nplanes, int variant, hipStream_t s);
 
 bool row_size_supported(int N);
 bool col_size_supported(int H);
+// Number of compiled FFT-plan variants for a row / column length (>= 1 when
+// supported).  Variant 0 is the default plan.
+int row_variants(int N);
+int col_variants(int H);
+
+// Plan variant chosen for each kernel kind.
+struct Variants {
+    int column = 0, ratio = 0, update = 0, ingest = 0, egress = 0;
+};
 
 }  // namespace fmvd

@@ -118,6 +118,21 @@ int fmvd_deconvolve_volume_device_cuda_8(const struct fmvd_plan_cuda_8 *plan,
                                          int iterations, const float *d_init, unsigned char *d_out,
                                          float *d_estimate, void *stream);
 
+/* Autotuner: times every compiled FFT-plan variant of each device kernel
+ * kind (0 column pass, 1 ratio row pass, 2 update row pass, 3 warm-start
+ * ingest, 4 egress) on this plan's buffers over `planes` planes (0 = 32)
+ * and keeps the fastest.  chosen[5] (may be NULL) receives the picks,
+ * times_ms[15] (may be NULL) the mean launch time of kind k, variant v at
+ * [3k + v] (negative where that variant does not exist for these extents).
+ * Deterministic results: every variant computes the same transform.  Not
+ * safe to call while the plan is running. */
+int fmvd_autotune_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes, int *chosen, float *times_ms);
+int fmvd_autotune_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes, int *chosen, float *times_ms);
+/* Sets the variants directly (variants[5] as above).  Returns -1 if a value is
+ * out of range for this plan's extents. */
+int fmvd_set_variants_cuda_16(struct fmvd_plan_cuda_16 *plan, const int *variants);
+int fmvd_set_variants_cuda_8(struct fmvd_plan_cuda_8 *plan, const int *variants);
+
 /* Plan geometry: fills dims[0..5] = dataH, dataW, fftH, fftW, nViews, batch planes. */
 int fmvd_plan_info_cuda_16(const struct fmvd_plan_cuda_16 *plan, int *dims);
 int fmvd_plan_info_cuda_8(const struct fmvd_plan_cuda_8 *plan, int *dims);

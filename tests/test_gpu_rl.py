@@ -84,6 +84,41 @@ def test_rl_parity(fm, orc, case):
     print(f"  fft={info['fft_h']}x{info['fft_w']} relL2={r:.2e} boundary flips={flips}", flush=True)
 
 
+VARIANT_CASES = [
+    (1, 2, 230, 250, 31, 31),     # 512 x 512
+    (1, 2, 600, 500, 41, 41),     # 1024 x 1024
+    (1, 2, 1024, 1024, 41, 41),   # 1536 x 1536
+    (1, 2, 2048, 300, 41, 41),    # 2560 x 512
+    (1, 2, 90, 2300, 41, 41),     # 128 x 2560 (row length 1280)
+]
+
+
+@pytest.mark.parametrize("case", VARIANT_CASES, ids=lambda c: f"{c[2]}x{c[3]}")
+def test_every_fft_plan_variant_matches_oracle(fm, orc, case):
+    """Each compiled FFT-plan variant (the autotuner's search space) computes
+    the same RL to the oracle's tolerance; the autotuner's pick is one of them."""
+    Z, V, dh, dw, kh, kw = case
+    data, kernels, weights = make_case(Z, V, dh, dw, kh, kw, 16, seed=dh + 3 * dw)
+    out_o, est_o = orc.rl_planes(data, weights, kernels.copy(), 0, 2, threads=8)
+    planes = [[data[z, v] for v in range(V)] for z in range(Z)]
+    with fm.Deconvolver(dh, dw, list(weights), [k.copy() for k in kernels]) as dec:
+        tuned = dec.autotune(planes=2)
+        print(f"\n[variants] {case} tuned={ {k: c for k, (c, _) in tuned.items()} }", flush=True)
+        for kind_idx, kind in enumerate(dec.KINDS):
+            nvar = sum(t is not None for t in tuned[kind][1])
+            for var in range(nvar):
+                vs = [0] * 5
+                vs[kind_idx] = var
+                dec.set_variants(vs)
+                if kind == "ingest":
+                    continue  # warm-start ingest is covered by test_warm_start_matches_oracle
+                out, est = dec.run(planes, 2, want_estimate=True)
+                check_rl(out, est, out_o, est_o)
+        dec.set_variants([c for c, _ in tuned.values()])
+        out, est = dec.run(planes, 2, want_estimate=True)
+        check_rl(out, est, out_o, est_o)
+
+
 def test_batches_and_device_volume_agree(fm):
     Z, V, dh, dw, k = 7, 2, 90, 110, 11
     data, kernels, weights = make_case(Z, V, dh, dw, k, k, 16, seed=3)
