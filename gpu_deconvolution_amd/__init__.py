@@ -122,6 +122,11 @@ def exported_symbols() -> list:
             "fmvd_deconvolve_files_cuda_8", "transform_cuda_16", "transform_cuda_8"]
 
 
+def restore_error_handler() -> None:
+    """Re-install the recording handler (JNI entry points install their own)."""
+    lib().setErrorHandler(_error_cb, None)
+
+
 def take_errors() -> list:
     out = list(_errors)
     _errors.clear()

@@ -138,7 +138,9 @@ void getCudaDeviceName(int dev, char* name)
         hipDeviceProp_t prop;
         FMVD_HIP(hipGetDeviceProperties(&prop, dev));
         memset(name, 0, 256);
-        strncpy(name, prop.name, 255);
+        // some ROCm builds leave the marketing name empty; the ISA name is
+        // then the most useful identifier
+        strncpy(name, prop.name[0] ? prop.name : prop.gcnArchName, 255);
     })
 }
 

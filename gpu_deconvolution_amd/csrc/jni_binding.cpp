@@ -126,6 +126,10 @@ void setJavaExceptionHandler(JNIEnv* env) { setErrorHandler(ThrowException, env)
 // Reads `count` floats from each of n files named by a Java String[].
 bool read_float_files(JNIEnv* env, jobjectArray names, int n, size_t count, std::vector<std::vector<float>>& out)
 {
+    if (n < 1 || names == nullptr || GetArrayLength(env, names) < n) {
+        ThrowException(env, "file name array holds fewer than nViews entries");
+        return false;
+    }
     out.assign(n, std::vector<float>(count));
     for (int v = 0; v < n; ++v) {
         jstring js = (jstring)GetObjectArrayElement(env, names, v);
@@ -215,7 +219,11 @@ void transform_impl(JNIEnv* env, jobjectArray data, jint w, jint h, jint d, jflo
         ThrowException(env, "invalid source dimensions");
         return;
     }
-    if (GetArrayLength(env, invMatrix) < 12) {
+    if (data == nullptr || GetArrayLength(env, data) < d) {
+        ThrowException(env, "data holds fewer than d planes");
+        return;
+    }
+    if (invMatrix == nullptr || GetArrayLength(env, invMatrix) < 12) {
         ThrowException(env, "invMatrix must hold 12 values");
         return;
     }
@@ -223,6 +231,11 @@ void transform_impl(JNIEnv* env, jobjectArray data, jint w, jint h, jint d, jflo
     std::vector<S*> cdata(d);
     for (int z = 0; z < d; ++z) {
         jdata[z] = (JArr)GetObjectArrayElement(env, data, z);
+        if (jdata[z] == nullptr || GetArrayLength(env, jdata[z]) < (long long)w * h) {
+            for (int k = 0; k < z; ++k) rel(env, jdata[k], cdata[k], JNI_ABORT);
+            ThrowException(env, "a plane holds fewer than w*h samples");
+            return;
+        }
         cdata[z] = (S*)get(env, jdata[z]);
         if (!cdata[z]) {
             for (int k = 0; k < z; ++k) rel(env, jdata[k], cdata[k], JNI_ABORT);
@@ -312,6 +325,10 @@ JNIEXPORT void JNICALL Java_fastspim_NativeSPIMReconstructionCuda_deconvolve(
     for (int v = 0; v < nViews; ++v) {
         kp[v] = kernels[v].data();
         wp[v] = weights[v].data();
+    }
+    if (inputfiles == nullptr || GetArrayLength(env, inputfiles) < nViews) {
+        ThrowException(env, "file name array holds fewer than nViews entries");
+        return;
     }
     std::vector<FILE*> files(nViews, nullptr);
     auto close_all = [&] {
