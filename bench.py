@@ -46,7 +46,27 @@ def parse():
     p.add_argument("--iterations", type=int, default=30)
     p.add_argument("--cpu-planes", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
-    return p.parse_args()
+    p.add_argument("--traffic-run", action="store_true",
+                   help="short run for HBM counter collection: 64 planes, 1 iteration, 1 step, no CPU "
+                        "baseline, then a 1 GiB device-to-device copy as the counters' calibration kernel")
+    args = p.parse_args()
+    if args.traffic_run:
+        args.planes, args.iterations, args.steps, args.warmup, args.no_cpu = 64, 1, 1, 0, True
+    return args
+
+
+def measured_traffic(fh, fw, kind, planes_per_launch):
+    """HBM bytes per launch of `kind` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per plane, calibrated against a plain copy), or
+    None when no summary exists for this FFT extent."""
+    path = os.path.join(ROOT, "profiles", "traffic_rl_kernels.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        per_plane = table[f"{fh}x{fw}"][kind]["bytes_per_plane"]
+    except (OSError, KeyError, ValueError):
+        return None
+    return per_plane * planes_per_launch
 
 
 def cpu_threads():
@@ -204,7 +224,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": dk["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(fh, fw, dom, planes_per_launch),
         },
         "rl_iteration_roofline": {"achieved": iter_gbs, "frac": iter_gbs / HBM_PEAK_GBS, "unit": "GB/s",
                                   "bytes_per_view_iter_plane": B_RL},
@@ -213,6 +233,17 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+    if args.traffic_run:
+        # calibration kernel for FETCH_SIZE / WRITE_SIZE: a plain 1 GiB copy
+        n = 1 << 30
+        a, b = fm.DeviceArray((n,), np.uint8), fm.DeviceArray((n,), np.uint8)
+        a.fill_zero()
+        fm.device_synchronize()
+        if synth.synth_lib().fmvd_synth_copy(a.ptr, b.ptr, n, None) != 0:
+            fm.check()
+        fm.device_synchronize()
+        log(f"[bench] calibration copy of {n} bytes done")
 
 
 if __name__ == "__main__":

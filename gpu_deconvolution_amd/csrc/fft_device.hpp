@@ -4,44 +4,61 @@
 // A length-N complex sequence is owned by T threads of one workgroup.  Each
 // thread holds P = N/T complex values in registers.  A pass of radix R makes
 // every thread run P/R radix-R butterflies on the values it holds; between
-// passes the values are exchanged through LDS (write outputs, barrier, read
-// the next pass's inputs, barrier).  The radix plan (R0, R1, ...) is a
-// compile-time list; every radix divides P.
+// passes the values are exchanged through LDS (write outputs, sync, read the
+// next pass's inputs, sync).  The radix plan (R0, R1, ...) is a compile-time
+// list; every radix divides P.
 //
-// Position convention of a Stockham pass (radix R, span NS = product of the
-// radices already applied, M = N/R butterflies): butterfly j reads positions
-// j + r*M (r < R) and writes positions (j/NS)*NS*R + j%NS + r*NS.  Thread t
-// owns butterflies j = t + b*T (b < P/R).  The input layout of the first pass
-// and the output layout of the last pass are therefore both "j + r*M", which
-// lets a kernel load straight from global memory into the first pass and
-// store straight from the last pass, and lets an inverse transform whose first
-// radix equals the forward transform's last radix start from the forward
-// transform's registers with no LDS round trip.
+// Forward pass i (radix R, span NS = R0*...*R(i-1), M = N/R butterflies):
+// butterfly j reads positions j + r*M (r < R), multiplies input r by
+// w^(k r) with w = exp(-2 pi i / (R NS)) and k = j mod NS, runs a radix-R DFT
+// and writes output r to position (j/NS)*NS*R + j%NS + r*NS.  Thread t owns
+// butterflies j = t + b*T (b < P/R).  The inverse transform runs the same
+// passes backwards and transposed: it reads where the forward pass wrote,
+// runs the inverse DFT, multiplies by the conjugate twiddles and writes where
+// the forward pass read.  Hence the forward's input layout ("j + r*N/R0") is
+// the inverse's output layout and the forward's output layout
+// ("j + r*N/R_last") is the inverse's input layout: a kernel can load straight
+// into the first pass, store straight from the last, and start an inverse
+// from the forward's registers.
 //
-// Twiddles of the Stockham passes come from a table tw[m] = exp(-2 pi i m/N)
-// (m < N) computed in double on the host; the inverse conjugates them.
+// Plans keep every pass that writes to LDS at power-of-two radix and span (a
+// radix with factors 3, 5 or 7 goes last), so every LDS address and twiddle
+// address of a pass is one per-thread base plus compile-time offsets; the
+// affine decomposition is verified at compile time (affine_out, affine_in,
+// affine_tw) and a generic path covers the rest.
+//
+// Twiddles come from per-pass tables computed in double on the host
+// (pass_twiddle_table), stored per pass as [r-1][k] so that the lanes of a
+// wavefront, which hold consecutive k, read consecutive addresses.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <vector>
+
+// Host-and-device inline: the pass code also runs in the host emulator that
+// checks every plan against a naive DFT.
+#define FMVD_HD __host__ __device__ __forceinline__
+
 namespace fmvd {
 
 // ---------------------------------------------------------------- complex ops
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+FMVD_HD float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+FMVD_HD float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+FMVD_HD float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // a * conj(b)
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+FMVD_HD float2 cmulc(float2 a, float2 b)
 {
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
-__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+FMVD_HD float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+FMVD_HD float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // multiply by (DIR * i): DIR = -1 -> -i, DIR = +1 -> +i
-template <int DIR> __device__ __forceinline__ float2 mul_i(float2 a)
+template <int DIR> FMVD_HD float2 mul_i(float2 a)
 {
     return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }
@@ -92,7 +109,7 @@ constexpr double ct_sin2pi(long m, long n) { return ct_cos2pi(m * 4 - n, 4 * n);
 
 // exp(DIR * 2 pi i m / n)
 template <int DIR>
-__device__ __forceinline__ constexpr float2 ct_twiddle(long m, long n)
+FMVD_HD constexpr float2 ct_twiddle(long m, long n)
 {
     return make_float2(float(ct_cos2pi(m, n)), float(DIR * ct_sin2pi(m, n)));
 }
@@ -103,11 +120,11 @@ __device__ __forceinline__ constexpr float2 ct_twiddle(long m, long n)
 template <int R, int DIR> struct Dft;
 
 template <int DIR> struct Dft<1, DIR> {
-    static __device__ __forceinline__ void run(float2*) {}
+    static FMVD_HD void run(float2*) {}
 };
 
 template <int DIR> struct Dft<2, DIR> {
-    static __device__ __forceinline__ void run(float2* v)
+    static FMVD_HD void run(float2* v)
     {
         float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -116,7 +133,7 @@ template <int DIR> struct Dft<2, DIR> {
 };
 
 template <int DIR> struct Dft<3, DIR> {
-    static __device__ __forceinline__ void run(float2* v)
+    static FMVD_HD void run(float2* v)
     {
         constexpr float c1 = -0.5f;
         constexpr float s1 = float(DIR) * 0.86602540378443864676f;
@@ -132,7 +149,7 @@ template <int DIR> struct Dft<3, DIR> {
 };
 
 template <int DIR> struct Dft<4, DIR> {
-    static __device__ __forceinline__ void run(float2* v)
+    static FMVD_HD void run(float2* v)
     {
         float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
         float2 t2 = cadd(v[1], v[3]), t3 = mul_i<DIR>(csub(v[1], v[3]));
@@ -144,7 +161,7 @@ template <int DIR> struct Dft<4, DIR> {
 };
 
 template <int DIR> struct Dft<5, DIR> {
-    static __device__ __forceinline__ void run(float2* v)
+    static FMVD_HD void run(float2* v)
     {
         constexpr float c1 = 0.30901699437494742410f;   // cos(2pi/5)
         constexpr float c2 = -0.80901699437494742410f;  // cos(4pi/5)
@@ -167,7 +184,7 @@ template <int DIR> struct Dft<5, DIR> {
 };
 
 template <int DIR> struct Dft<7, DIR> {
-    static __device__ __forceinline__ void run(float2* v)
+    static FMVD_HD void run(float2* v)
     {
         constexpr float c1 = 0.62348980185873353053f, c2 = -0.22252093395631440429f,
                         c3 = -0.90096886790241912624f;
@@ -197,7 +214,7 @@ template <int DIR> struct Dft<7, DIR> {
 // Composite R = A*B by in-register Cooley-Tukey:
 //   X[k1 + A*k2] = sum_n2 W_R^{n2 k1} [sum_n1 x[B*n1 + n2] W_A^{n1 k1}] W_B^{n2 k2}
 template <int A, int B, int DIR> struct DftSplit {
-    static __device__ __forceinline__ void run(float2* v)
+    static FMVD_HD void run(float2* v)
     {
         constexpr int R = A * B;
         float2 t[R];
@@ -267,15 +284,15 @@ template <> struct Plan<64> : PlanDef<64, 4, RadixList<16, 4>> {};
 template <> struct Plan<128> : PlanDef<128, 8, RadixList<16, 8>> {};
 template <> struct Plan<256> : PlanDef<256, 16, RadixList<16, 16>> {};
 template <> struct Plan<512> : PlanDef<512, 32, RadixList<16, 16, 2>> {};
-template <> struct Plan<768> : PlanDef<768, 32, RadixList<12, 8, 8>> {};
+template <> struct Plan<768> : PlanDef<768, 32, RadixList<8, 8, 12>> {};
 template <> struct Plan<1024> : PlanDef<1024, 64, RadixList<16, 16, 4>> {};
-template <> struct Plan<1280> : PlanDef<1280, 64, RadixList<20, 4, 4, 4>> {};
-template <> struct Plan<1536> : PlanDef<1536, 64, RadixList<24, 8, 8>> {};
-template <> struct Plan<1792> : PlanDef<1792, 64, RadixList<28, 4, 4, 4>> {};
+template <> struct Plan<1280> : PlanDef<1280, 64, RadixList<4, 4, 4, 20>> {};
+template <> struct Plan<1536> : PlanDef<1536, 64, RadixList<8, 8, 24>> {};
+template <> struct Plan<1792> : PlanDef<1792, 64, RadixList<4, 4, 4, 28>> {};
 template <> struct Plan<2048> : PlanDef<2048, 128, RadixList<16, 16, 8>> {};
-template <> struct Plan<2560> : PlanDef<2560, 64, RadixList<5, 8, 8, 8>> {};
-template <> struct Plan<3072> : PlanDef<3072, 64, RadixList<12, 16, 16>> {};
-template <> struct Plan<3584> : PlanDef<3584, 64, RadixList<7, 8, 8, 8>> {};
+template <> struct Plan<2560> : PlanDef<2560, 64, RadixList<8, 8, 8, 5>> {};
+template <> struct Plan<3072> : PlanDef<3072, 64, RadixList<16, 16, 12>> {};
+template <> struct Plan<3584> : PlanDef<3584, 64, RadixList<8, 8, 8, 7>> {};
 template <> struct Plan<4096> : PlanDef<4096, 256, RadixList<16, 16, 16>> {};
 
 // Alternative plans for the extents of the benchmark workloads, selected at
@@ -284,22 +301,24 @@ template <> struct Plan<4096> : PlanDef<4096, 256, RadixList<16, 16, 16>> {};
 constexpr int kPlanVariants = 3;
 template <int N, int V> struct PlanV : Plan<N> {
     static constexpr bool distinct = (V == 0);
+    static constexpr int variant = 0;
 };
 #define FMVD_PLAN_VARIANT(n, v, t, ...)                                          \
     template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>> {     \
         static constexpr bool distinct = true;                                   \
+        static constexpr int variant = v;                                        \
     };
 FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
 FMVD_PLAN_VARIANT(512, 1, 64, 8, 8, 8)
-FMVD_PLAN_VARIANT(768, 1, 64, 12, 4, 4, 4)
-FMVD_PLAN_VARIANT(768, 2, 64, 3, 4, 4, 4, 4)
+FMVD_PLAN_VARIANT(768, 1, 64, 4, 4, 4, 12)
+FMVD_PLAN_VARIANT(768, 2, 64, 4, 4, 4, 4, 3)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
-FMVD_PLAN_VARIANT(1280, 1, 32, 20, 8, 8)
-FMVD_PLAN_VARIANT(1280, 2, 64, 5, 4, 4, 4, 4)
-FMVD_PLAN_VARIANT(1536, 1, 64, 3, 8, 8, 8)
-FMVD_PLAN_VARIANT(1536, 2, 64, 12, 4, 4, 8)
-FMVD_PLAN_VARIANT(2560, 1, 64, 10, 4, 8, 8)
+FMVD_PLAN_VARIANT(1280, 1, 32, 8, 8, 20)
+FMVD_PLAN_VARIANT(1280, 2, 64, 4, 4, 4, 4, 5)
+FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
+FMVD_PLAN_VARIANT(1536, 2, 64, 4, 4, 8, 12)
+FMVD_PLAN_VARIANT(2560, 1, 64, 4, 8, 8, 10)
 #undef FMVD_PLAN_VARIANT
 
 // A plan is valid when its radices multiply to N and each divides P.
@@ -320,76 +339,183 @@ template <typename PL> constexpr int span_before(int i)
     for (int k = 0; k < i; ++k) s *= PL::R::at(k);
     return s;
 }
-// radix of forward pass i / inverse pass i (inverse runs the list reversed)
+// Radix of step i of direction DIR, in the order the steps run: the forward
+// runs passes 0..NP-1, the inverse runs them NP-1..0.
 template <typename PL, int DIR> constexpr int radix_of(int i)
 {
     return DIR < 0 ? PL::R::at(i) : PL::R::at(PL::NP - 1 - i);
 }
-template <typename PL, int DIR> constexpr int ns_of(int i)
+// Radix whose "j + r*N/R" layout a direction consumes / produces.
+template <typename PL, int DIR> constexpr int first_radix() { return radix_of<PL, DIR>(0); }
+template <typename PL, int DIR> constexpr int last_radix() { return radix_of<PL, DIR>(PL::NP - 1); }
+
+// Offset of forward pass i's twiddles in the plan's table (pass 0 has NS = 1
+// and none); the inverse uses the same entries conjugated.
+template <typename PL> constexpr int tw_pass_offset(int i)
 {
-    int s = 1;
-    for (int k = 0; k < i; ++k) s *= radix_of<PL, DIR>(k);
-    return s;
+    int off = 0;
+    for (int q = 1; q < i; ++q) off += (PL::R::at(q) - 1) * span_before<PL>(q);
+    return off;
+}
+template <typename PL> constexpr int tw_table_len() { return tw_pass_offset<PL>(PL::NP); }
+
+// Host: the per-pass twiddle table of plan PL.
+template <typename PL> std::vector<float2> pass_twiddle_table()
+{
+    std::vector<float2> t((size_t)tw_table_len<PL>());
+    for (int q = 1; q < PL::NP; ++q) {
+        const int R = PL::R::at(q), NS = span_before<PL>(q), off = tw_pass_offset<PL>(q);
+        const double m = (double)R * NS;
+        for (int r = 1; r < R; ++r)
+            for (int k = 0; k < NS; ++k) {
+                // exact argument reduction on the integer k*r mod (R*NS)
+                const long e = ((long)k * r) % (long)(R * NS);
+                const double a = -2.0 * 3.14159265358979323846 * (double)e / m;
+                t[(size_t)off + (size_t)(r - 1) * NS + k] = make_float2((float)std::cos(a), (float)std::sin(a));
+            }
+    }
+    return t;
 }
 
 // LDS index with one pad slot every 16 complex values (bank spreading).
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+FMVD_HD constexpr int lds_pad(int i) { return i + (i >> 4); }
 template <int N> constexpr int lds_len() { return N + N / 16; }
 
-// Butterflies of one pass held in registers: v[b*R + r] is input r of
-// butterfly j = t + b*T (on entry) and output r (on exit).
-template <typename PL, int R, int NS, int DIR>
-__device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __restrict__ tw)
+// Positions of forward pass I (see the header comment).
+template <typename PL, int I> FMVD_HD constexpr int in_pos_p(int t, int b, int r)
 {
-    constexpr int N = PL::N, T = PL::T, P = PL::P;
-    constexpr int NB = P / R;
-    constexpr int TSTEP = N / (NS * R);
+    return t + b * PL::T + r * (PL::N / PL::R::at(I));
+}
+template <typename PL, int I> FMVD_HD constexpr int out_pos_p(int t, int b, int r)
+{
+    constexpr int R = PL::R::at(I), NS = span_before<PL>(I);
+    const int j = t + b * PL::T;
+    return (j / NS) * NS * R + j % NS + r * NS;
+}
+template <typename PL, int I> FMVD_HD constexpr int tw_k(int t, int b)
+{
+    return (t + b * PL::T) % span_before<PL>(I);
+}
+
+// Affine decompositions, verified exhaustively at compile time:
+//   lds_pad(pos(t, b, r)) == lds_pad(pos(t, 0, 0)) + lds_pad(pos(0, b, r))
+//   tw_k(t, b) == tw_k(t, 0) + tw_k(0, b)
+template <typename PL, int I> constexpr bool affine_out()
+{
+    for (int t = 0; t < PL::T; ++t)
+        for (int b = 0; b < PL::P / PL::R::at(I); ++b)
+            for (int r = 0; r < PL::R::at(I); ++r)
+                if (lds_pad(out_pos_p<PL, I>(t, b, r)) !=
+                    lds_pad(out_pos_p<PL, I>(t, 0, 0)) + lds_pad(out_pos_p<PL, I>(0, b, r)))
+                    return false;
+    return true;
+}
+template <typename PL, int I> constexpr bool affine_in()
+{
+    for (int t = 0; t < PL::T; ++t)
+        for (int b = 0; b < PL::P / PL::R::at(I); ++b)
+            for (int r = 0; r < PL::R::at(I); ++r)
+                if (lds_pad(in_pos_p<PL, I>(t, b, r)) != lds_pad(t) + lds_pad(in_pos_p<PL, I>(0, b, r))) return false;
+    return true;
+}
+template <typename PL, int I> constexpr bool affine_tw()
+{
+    for (int t = 0; t < PL::T; ++t)
+        for (int b = 0; b < PL::P / PL::R::at(I); ++b)
+            if (tw_k<PL, I>(t, b) != tw_k<PL, I>(t, 0) + tw_k<PL, I>(0, b)) return false;
+    return true;
+}
+
+// LDS traffic of pass I.  "in" slots are the positions the forward reads
+// (and the inverse writes), "out" slots those the forward writes (and the
+// inverse reads).  v[b*R + r] is slot (b, r).
+template <typename PL, int I, bool STORE>
+FMVD_HD void lds_in_slots(float2* lds, float2* v, int t)
+{
+    constexpr int R = PL::R::at(I), NB = PL::P / R;
+    if constexpr (affine_in<PL, I>()) {
+        float2* base = lds + lds_pad(t);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int j = t + b * T;
-        if (NS > 1) {
-            const int k = j % NS;
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                float2 w = tw[k * r * TSTEP];
-                v[b * R + r] = DIR < 0 ? cmul(v[b * R + r], w) : cmulc(v[b * R + r], w);
+            for (int r = 0; r < R; ++r) {
+                float2& x = base[lds_pad(in_pos_p<PL, I>(0, b, r))];
+                if constexpr (STORE) x = v[b * R + r];
+                else v[b * R + r] = x;
             }
-        }
-        Dft<R, DIR>::run(v + b * R);
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float2& x = lds[lds_pad(in_pos_p<PL, I>(t, b, r))];
+                if constexpr (STORE) x = v[b * R + r];
+                else v[b * R + r] = x;
+            }
+    }
+}
+template <typename PL, int I, bool STORE>
+FMVD_HD void lds_out_slots(float2* lds, float2* v, int t)
+{
+    constexpr int R = PL::R::at(I), NB = PL::P / R;
+    if constexpr (affine_out<PL, I>()) {
+        float2* base = lds + lds_pad(out_pos_p<PL, I>(t, 0, 0));
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float2& x = base[lds_pad(out_pos_p<PL, I>(0, b, r))];
+                if constexpr (STORE) x = v[b * R + r];
+                else v[b * R + r] = x;
+            }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float2& x = lds[lds_pad(out_pos_p<PL, I>(t, b, r))];
+                if constexpr (STORE) x = v[b * R + r];
+                else v[b * R + r] = x;
+            }
     }
 }
 
-// Position of input r of butterfly b for thread t in pass with radix R.
-template <typename PL, int R>
-__device__ __forceinline__ int in_pos(int t, int b, int r)
+// Twiddles of pass I: forward multiplies input r by w^(k r), the inverse
+// multiplies output r of its inverse DFT by the conjugate.
+template <typename PL, int I, bool CONJ>
+FMVD_HD void pass_twiddle(float2* v, int t, const float2* __restrict__ tw)
 {
-    return t + b * PL::T + r * (PL::N / R);
-}
-// Position of output r of butterfly b for thread t in pass (R, NS).
-template <typename PL, int R, int NS>
-__device__ __forceinline__ int out_pos(int t, int b, int r)
-{
-    const int j = t + b * PL::T;
-    return (j / NS) * NS * R + (j % NS) + r * NS;
+    constexpr int R = PL::R::at(I), NB = PL::P / R, NS = span_before<PL>(I);
+    if constexpr (NS > 1) {
+        const float2* twp = tw + tw_pass_offset<PL>(I);
+        if constexpr (affine_tw<PL, I>()) {
+            const float2* tb = twp + tw_k<PL, I>(t, 0);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = tb[tw_k<PL, I>(0, b) + (r - 1) * NS];
+                    v[b * R + r] = CONJ ? cmulc(v[b * R + r], w) : cmul(v[b * R + r], w);
+                }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float2* tb = twp + tw_k<PL, I>(t, b);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = tb[(r - 1) * NS];
+                    v[b * R + r] = CONJ ? cmulc(v[b * R + r], w) : cmul(v[b * R + r], w);
+                }
+            }
+        }
+    }
 }
 
-template <typename PL, int R, int NS>
-__device__ __forceinline__ void lds_store_out(float2* lds, const float2* v, int t)
+template <typename PL, int I, int DIR> FMVD_HD void pass_dft(float2* v)
 {
-    constexpr int NB = PL::P / R;
+    constexpr int R = PL::R::at(I), NB = PL::P / R;
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_pad(out_pos<PL, R, NS>(t, b, r))] = v[b * R + r];
-}
-template <typename PL, int R>
-__device__ __forceinline__ void lds_load_in(const float2* lds, float2* v, int t)
-{
-    constexpr int NB = PL::P / R;
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b * R + r] = lds[lds_pad(in_pos<PL, R>(t, b, r))];
+    for (int b = 0; b < NB; ++b) Dft<R, DIR>::run(v + b * R);
 }
 
 // Synchronisation of the threads that own one sequence.  When a sequence's T
@@ -409,50 +535,58 @@ template <typename PL> __device__ __forceinline__ void seq_sync()
     }
 }
 
-// Runs passes [I, NP) of a transform whose pass I-1 has already been computed
-// in registers (or I == 0 with the first pass's inputs in registers).  The
-// exchange between passes goes through `lds` (this sequence's slice); all T
-// threads of the sequence must call this.
-template <typename PL, int DIR, int I>
-__device__ __forceinline__ void run_passes_from(float2* v, float2* lds, int t, const float2* __restrict__ tw)
+// Forward passes I..NP-1; on entry v holds pass I-1's outputs (or, for I = 0,
+// the inputs at the "j + r*N/R0" layout).
+template <typename PL, int I>
+__device__ __forceinline__ void fwd_passes(float2* v, float2* lds, int t, const float2* __restrict__ tw)
 {
     if constexpr (I < PL::NP) {
-        constexpr int R = radix_of<PL, DIR>(I);
-        constexpr int NS = ns_of<PL, DIR>(I);
         if constexpr (I > 0) {
-            constexpr int RP = radix_of<PL, DIR>(I - 1);
-            constexpr int NSP = ns_of<PL, DIR>(I - 1);
-            lds_store_out<PL, RP, NSP>(lds, v, t);
+            lds_out_slots<PL, I - 1, true>(lds, v, t);
             seq_sync<PL>();
-            lds_load_in<PL, R>(lds, v, t);
+            lds_in_slots<PL, I, false>(lds, v, t);
             seq_sync<PL>();
         }
-        pass_compute<PL, R, NS, DIR>(v, t, tw);
-        run_passes_from<PL, DIR, I + 1>(v, lds, t, tw);
+        pass_twiddle<PL, I, false>(v, t, tw);
+        pass_dft<PL, I, -1>(v);
+        fwd_passes<PL, I + 1>(v, lds, t, tw);
+    }
+}
+// Inverse passes I..0 (transposed forward passes, run backwards); on entry v
+// holds the values at pass I's "out" slots.
+template <typename PL, int I>
+__device__ __forceinline__ void inv_passes(float2* v, float2* lds, int t, const float2* __restrict__ tw)
+{
+    if constexpr (I >= 0) {
+        if constexpr (I < PL::NP - 1) {
+            lds_in_slots<PL, I + 1, true>(lds, v, t);
+            seq_sync<PL>();
+            lds_out_slots<PL, I, false>(lds, v, t);
+            seq_sync<PL>();
+        }
+        pass_dft<PL, I, +1>(v);
+        pass_twiddle<PL, I, true>(v, t, tw);
+        inv_passes<PL, I - 1>(v, lds, t, tw);
     }
 }
 
-// Full transform, first-pass inputs already in registers at in_pos of the
-// first radix; on exit the outputs are in registers at in_pos of the last
-// radix of this direction (natural-order position p = j + r*N/R_last).
-// The caller must have ordered any earlier LDS reads of this sequence's slice
-// before the call (seq_sync or a workgroup barrier).
+// Full unnormalised transform of the sequence owned by T threads, kernel
+// exp(DIR 2 pi i n k / N).  On entry v holds the inputs at the layout
+// "j + r*N/R" of R = first_radix<PL, DIR>() (slot_pos); on exit the outputs,
+// in natural order, at the layout of last_radix<PL, DIR>().  The caller must
+// have ordered any earlier LDS reads of this sequence's slice before the
+// call (seq_sync or a workgroup barrier).
 template <typename PL, int DIR>
 __device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, const float2* __restrict__ tw)
 {
-    run_passes_from<PL, DIR, 0>(v, lds, t, tw);
+    if constexpr (DIR < 0) fwd_passes<PL, 0>(v, lds, t, tw);
+    else inv_passes<PL, PL::NP - 1>(v, lds, t, tw);
 }
 
-// Radix of the first pass of direction DIR (defines the register layout on
-// entry) and of the last pass (defines the layout on exit).
-template <typename PL, int DIR> constexpr int first_radix() { return radix_of<PL, DIR>(0); }
-template <typename PL, int DIR> constexpr int last_radix() { return radix_of<PL, DIR>(PL::NP - 1); }
+// Position of input r of butterfly b for thread t in the layout of radix R.
+template <typename PL, int R> FMVD_HD int in_pos(int t, int b, int r) { return t + b * PL::T + r * (PL::N / R); }
 
 // Natural-order position of register slot i (= b*R + r) for a layout "j + r*N/R".
-template <typename PL, int R>
-__device__ __forceinline__ int slot_pos(int t, int i)
-{
-    return in_pos<PL, R>(t, i / R, i % R);
-}
+template <typename PL, int R> FMVD_HD int slot_pos(int t, int i) { return in_pos<PL, R>(t, i / R, i % R); }
 
 }  // namespace fmvd

@@ -35,6 +35,10 @@ namespace fmvd {
 
 void throw_hip(hipError_t e, const char* what, const char* file, int line)
 {
+    // The runtime keeps the last error of any call until hipGetLastError reads
+    // it; this error is reported now, so clear it here or the next launch
+    // check would report it again.
+    (void)hipGetLastError();
     char msg[512];
     const char* base = strrchr(file, '/');
     snprintf(msg, sizeof msg, "HIP error %d: %s in %s (line %i): %s", (int)e, hipGetErrorString(e),

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS) void col_conv_kernel(ColGeom g
     float2 v[C::P];
 #pragma unroll
     for (int i = 0; i < C::P; ++i) v[i] = xin[slot_pos<PL, C::RF>(t, i)];
-    fft_regs<PL, -1>(v, row, t, g.twH);
+    fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
 
     if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
         if (seq == 0) {
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS) void col_conv_kernel(ColGeom g
         for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[slot_pos<PL, C::RL>(t, i)]);
     }
 
-    fft_regs<PL, +1>(v, row, t, g.twH);
+    fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);
     float2* xo = Xout + p * xstride + (long)kx * H;
 #pragma unroll
     for (int i = 0; i < C::P; ++i) xo[slot_pos<PL, C::RF>(t, i)] = v[i];
@@ -164,6 +164,25 @@ static void col_variant(int v, const ColGeom& g, const float2* Xin, float2* Xout
     } else {
         launch_col_plan<PlanV<H, 0>>(g, Xin, Xout, xstride, K, nplanes, s);
     }
+}
+
+std::vector<float2> col_pass_twiddles(int H, int variant)
+{
+    std::vector<float2> t;
+    auto get = [&]<typename PL>() { t = pass_twiddle_table<PL>(); };
+    switch (H) {
+#define FMVD_CASE(n)                                                                   \
+    case n:                                                                            \
+        if (variant == 1 && PlanV<n, 1>::distinct) get.template operator()<PlanV<n, 1>>(); \
+        else if (variant == 2 && PlanV<n, 2>::distinct) get.template operator()<PlanV<n, 2>>(); \
+        else get.template operator()<PlanV<n, 0>>();                                   \
+        break;
+        FMVD_COL_SIZES(FMVD_CASE)
+#undef FMVD_CASE
+    default:
+        throw_msg("unsupported column FFT length");
+    }
+    return t;
 }
 
 void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K, int nplanes,

@@ -9,8 +9,9 @@
 //   A  column pass   XT = colconv(XE, K_v / H)                (spectrum of Q)
 //   B  row pass      XT = R2C(Q > 1 ? D_v / Q : Q), Q = C2R(XT)
 //   C  column pass   XT = colconv(XT, Khat_v / (H W))          (spectrum of C)
-//   D  row pass      E = C2R(XE), C = C2R(XT), E += w_v (E C - E), XE = R2C(E)/W
-// XE is the row spectrum of the estimate scaled by 1/W, so C2R(XE) = E.
+//   D  row pass      C = C2R(XT), E += w_v (E C - E), XE = R2C(E)/W
+// E (ES) is the spatial estimate on the padded plane, as in the reference; XE
+// is its row spectrum scaled by 1/W, kept so that pass A needs no row FFT.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -102,7 +103,7 @@ struct Event {
 template <typename T> static void upload(DeviceBuffer& b, const std::vector<T>& v)
 {
     b.alloc(sizeof(T) * v.size());
-    FMVD_HIP(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    if (!v.empty()) FMVD_HIP(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
 }
 
 static size_t env_bytes(const char* name, size_t dflt)
@@ -127,13 +128,14 @@ template <typename S> struct Plan {
     int batchCap = 0;
     Variants var;
 
-    DeviceBuffer twN, twW, twH;
+    DeviceBuffer twW, twRow[3], twCol[3];
     std::vector<DeviceBuffer> K, Kh, Wn;
-    DeviceBuffer XE, XT;
+    DeviceBuffer XE, XT, ES;
     std::unique_ptr<Stream> compute, copy;
 
     size_t plane() const { return (size_t)dataH * dataW; }
     size_t xplane() const { return (size_t)N * fftH; }  // complex per plane
+    size_t eplane() const { return (size_t)fftW * fftH; }  // floats per plane
 
     RowGeom row_geom() const
     {
@@ -150,7 +152,7 @@ template <typename S> struct Plan {
         g.dkX = kernelH / 2;
         g.wkY = kernelH / 2;
         g.wkX = kernelW / 2;
-        g.twN = twN.as<float2>();
+        for (int v = 0; v < 3; ++v) g.twP[v] = twRow[v].as<float2>();
         g.twW = twW.as<float2>();
         return g;
     }
@@ -159,7 +161,7 @@ template <typename S> struct Plan {
         ColGeom g;
         g.H = fftH;
         g.N = N;
-        g.twH = twH.as<float2>();
+        for (int v = 0; v < 3; ++v) g.twP[v] = twCol[v].as<float2>();
         return g;
     }
 
@@ -167,7 +169,7 @@ template <typename S> struct Plan {
     {
         if (batchCap > 0) return batchCap;
         const size_t budget = env_bytes("FMVD_BATCH_BYTES", (size_t)2 << 30);
-        const size_t per = 2 * xplane() * sizeof(float2);
+        const size_t per = 2 * xplane() * sizeof(float2) + eplane() * sizeof(float);
         size_t nb = budget / (per ? per : 1);
         nb = std::max<size_t>(1, std::min<size_t>(nb, 4096));
         return (int)std::max<size_t>(nb, (size_t)std::max(1, nStreams));
@@ -177,6 +179,7 @@ template <typename S> struct Plan {
     {
         XE.alloc(xplane() * sizeof(float2) * (size_t)planes);
         XT.alloc(xplane() * sizeof(float2) * (size_t)planes);
+        ES.alloc(eplane() * sizeof(float) * (size_t)planes);
     }
 
     void init(int dH, int dW, const float* const* h_Weights, float** h_Kernel, int kH, int kW,
@@ -213,9 +216,12 @@ template <typename S> struct Plan {
         std::vector<double> khat = compute_kernel_hat(h_Kernel, kernelW, kernelH, nViews, psf);
         for (int v = 0; v < nViews; ++v) normalize_kernel_f32(h_Kernel[v], len);
 
-        upload(twN, twiddles(N, N));
         upload(twW, twiddles(fftW, N));
-        upload(twH, twiddles(fftH, fftH));
+        for (int v = 0; v < 3; ++v) {
+            // variants that do not exist fall back to plan 0 (same table)
+            upload(twRow[v], row_pass_twiddles(N, v));
+            upload(twCol[v], col_pass_twiddles(fftH, v));
+        }
 
         K.resize(nViews);
         Kh.resize(nViews);
@@ -246,10 +252,12 @@ template <typename S> struct Plan {
         const long xs = (long)xplane();
         float2* xe = XE.as<float2>();
         float2* xt = XT.as<float2>();
+        float* es = ES.as<float>();
+        const long esd = (long)eplane();
         {
             TimedScope ts(kTimeIngest, s);
-            if (init) launch_row_init(rg, xe, xs, init, istride, np, var.ingest, s);
-            else launch_const_init(rg, xe, xs, 10.0f * (float)nViews, np, s);
+            if (init) launch_row_init(rg, xe, es, xs, esd, init, istride, np, var.ingest, s);
+            else launch_const_init(rg, xe, es, xs, esd, 10.0f * (float)nViews, np, s);
         }
         for (int it = 0; it < iterations; ++it) {
             for (int v = 0; v < nViews; ++v) {
@@ -267,12 +275,12 @@ template <typename S> struct Plan {
                 }
                 {
                     TimedScope ts(kTimeUpdate, s);
-                    launch_row_update(rg, xe, xt, xs, Wn[v].as<float>(), np, var.update, s);
+                    launch_row_update(rg, xe, xt, es, xs, esd, Wn[v].as<float>(), np, var.update, s);
                 }
             }
         }
         TimedScope ts(kTimeEgress, s);
-        launch_row_final<S>(rg, xe, xs, out, est, ostride, np, var.egress, s);
+        launch_final<S>(rg, es, esd, out, est, ostride, np, s);
     }
 
     // Autotuner: times every compiled FFT-plan variant of each kernel kind on
@@ -297,7 +305,9 @@ template <typename S> struct Plan {
         const long xs = (long)xplane();
         float2* xe = XE.as<float2>();
         float2* xt = XT.as<float2>();
-        launch_const_init(rg, xe, xs, 10.0f * (float)nViews, planes, s);
+        float* es = ES.as<float>();
+        const long esd = (long)eplane();
+        launch_const_init(rg, xe, es, xs, esd, 10.0f * (float)nViews, planes, s);
         launch_col_conv(cg, xe, xt, xs, K[0].as<float2>(), planes, 0, s);
         hipEvent_t a, b;
         FMVD_HIP(hipEventCreate(&a));
@@ -316,7 +326,7 @@ template <typename S> struct Plan {
         const int nc = col_variants(fftH), nr = row_variants(N);
         int* slots[5] = {&var.column, &var.ratio, &var.update, &var.ingest, &var.egress};
         for (int k = 0; k < 5; ++k) {
-            const int nv = k == 0 ? nc : nr;
+            const int nv = k == 0 ? nc : (k == 4 ? 1 : nr);
             float best = 0.f;
             int bestv = 0;
             for (int v = 0; v < 3; ++v) {
@@ -325,9 +335,9 @@ template <typename S> struct Plan {
                     switch (k) {
                     case 0: ms = time_it([&] { launch_col_conv(cg, xt, xt, xs, Kh[0].as<float2>(), planes, v, s); }); break;
                     case 1: ms = time_it([&] { launch_row_ratio<S>(rg, xt, xs, data.as<S>(), (long)pl, planes, v, s); }); break;
-                    case 2: ms = time_it([&] { launch_row_update(rg, xe, xt, xs, Wn[0].as<float>(), planes, v, s); }); break;
-                    case 3: ms = time_it([&] { launch_row_init(rg, xt, xs, e0.as<float>(), (long)pl, planes, v, s); }); break;
-                    case 4: ms = time_it([&] { launch_row_final<S>(rg, xe, xs, out.as<S>(), nullptr, (long)pl, planes, v, s); }); break;
+                    case 2: ms = time_it([&] { launch_row_update(rg, xe, xt, es, xs, esd, Wn[0].as<float>(), planes, v, s); }); break;
+                    case 3: ms = time_it([&] { launch_row_init(rg, xt, es, xs, esd, e0.as<float>(), (long)pl, planes, v, s); }); break;
+                    case 4: ms = time_it([&] { launch_final<S>(rg, es, esd, out.as<S>(), nullptr, (long)pl, planes, s); }); break;
                     }
                     if (v == 0 || ms < best) {
                         best = ms;

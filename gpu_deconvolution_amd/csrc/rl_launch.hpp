@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace fmvd {
 
 struct RowGeom {
@@ -22,13 +24,13 @@ struct RowGeom {
     int dataH, dataW;
     int dkY, dkX;         // data pad half-widths (kernelW/2, kernelH/2: swapped as the reference)
     int wkY, wkX;         // weight pad half-widths (kernelH/2, kernelW/2)
-    const float2* twN;    // exp(-2 pi i m / N), m < N
+    const float2* twP[3]; // per-pass twiddle tables of the row FFT plan variants 0..2
     const float2* twW;    // exp(-2 pi i p / W), p < N
 };
 
 struct ColGeom {
     int H, N;
-    const float2* twH;    // exp(-2 pi i m / H), m < H
+    const float2* twP[3]; // per-pass twiddle tables of the column FFT plan variants 0..2
 };
 
 // Pass B: X <- R2C(ratio(D, C2R(X))) for nplanes planes, in place.
@@ -36,22 +38,27 @@ struct ColGeom {
 template <typename SAMPLE>
 void launch_row_ratio(const RowGeom& g, float2* X, long xstride, const SAMPLE* data, long dstride,
                       int nplanes, int variant, hipStream_t s);
-// Pass D: E = C2R(XE); C = C2R(XC); E += w*(E*C - E); XE <- R2C(E)/W.
-void launch_row_update(const RowGeom& g, float2* XE, const float2* XC, long xstride,
+// Pass D: C = C2R(XC); E += w*(E*C - E) on the spatial estimate ES (fftH x
+// fftW floats per plane, row-major); XE <- R2C(E)/W.
+void launch_row_update(const RowGeom& g, float2* XE, const float2* XC, float* ES, long xstride, long estride,
                        const float* weights, int nplanes, int variant, hipStream_t s);
-// Egress: E = C2R(XE) on rows < dataH; out = sample(E), est = E (either may be null).
+// Egress: crop ES to dataH x dataW; out = sample(E), est = E (either may be null).
 template <typename SAMPLE>
-void launch_row_final(const RowGeom& g, const float2* XE, long xstride, SAMPLE* out, float* est,
-                      long ostride, int nplanes, int variant, hipStream_t s);
-// Warm-start ingest: XE <- R2C(pad(E0))/W, E0 is dataH x dataW per plane.
-void launch_row_init(const RowGeom& g, float2* XE, long xstride, const float* e0, long estride,
-                     int nplanes, int variant, hipStream_t s);
-// Constant start E0 = value everywhere: XE[0][y] = (value, 0), rest 0.
-void launch_const_init(const RowGeom& g, float2* XE, long xstride, float value, int nplanes,
-                       hipStream_t s);
+void launch_final(const RowGeom& g, const float* ES, long estride, SAMPLE* out, float* est, long ostride,
+                  int nplanes, hipStream_t s);
+// Warm-start ingest: ES <- pad(E0), XE <- R2C(ES)/W; E0 is dataH x dataW per plane.
+void launch_row_init(const RowGeom& g, float2* XE, float* ES, long xstride, long estride, const float* e0,
+                     long e0stride, int nplanes, int variant, hipStream_t s);
+// Constant start E0 = value everywhere: ES = value, XE[0][y] = (value, 0), rest 0.
+void launch_const_init(const RowGeom& g, float2* XE, float* ES, long xstride, long estride, float value,
+                       int nplanes, hipStream_t s);
 // Passes A / C: Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]) for every kx.
 void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride,
                      const float2* K, int nplanes, int variant, hipStream_t s);
+
+// Per-pass twiddle tables (host side) of a row / column FFT plan variant.
+std::vector<float2> row_pass_twiddles(int N, int variant);
+std::vector<float2> col_pass_twiddles(int H, int variant);
 
 bool row_size_supported(int N);
 bool col_size_supported(int H);

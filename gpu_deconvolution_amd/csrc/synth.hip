@@ -126,6 +126,14 @@ __global__ void poisson_kernel(const float* __restrict__ lam, S* __restrict__ ou
     out[i] = (S)(k > maxv ? maxv : k);
 }
 
+// Plain streaming copy (16 bytes per thread): the known-size reference kernel
+// for HBM byte counters (reads n bytes, writes n bytes).
+__global__ void copy_kernel(const uint4* __restrict__ in, uint4* __restrict__ out, long n16)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) out[i] = in[i];
+}
+
 inline unsigned blocks(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -179,6 +187,18 @@ int fmvd_synth_poisson(const float* lam, void* out, long n, unsigned long long s
             poisson_kernel<uint16_t><<<blocks(n, 256), 256, 0, s>>>(lam, (uint16_t*)out, n, seed, 65535.0);
         else
             poisson_kernel<uint8_t><<<blocks(n, 256), 256, 0, s>>>(lam, (uint8_t*)out, n, seed, 255.0);
+        FMVD_LAUNCH_CHECK();
+        return 0;
+    })
+}
+
+// bytes must be a multiple of 16; both pointers 16-byte aligned.
+int fmvd_synth_copy(const void* in, void* out, long bytes, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (bytes < 0 || bytes % 16) fmvd::throw_msg("fmvd_synth_copy: bytes must be a non-negative multiple of 16");
+        const long n16 = bytes / 16;
+        if (n16) copy_kernel<<<blocks(n16, 256), 256, 0, (hipStream_t)stream>>>((const uint4*)in, (uint4*)out, n16);
         FMVD_LAUNCH_CHECK();
         return 0;
     })

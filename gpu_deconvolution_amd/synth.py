@@ -140,6 +140,7 @@ def synth_lib():
         L.fmvd_synth_truth.argtypes = [vp, i, i, i, vp, i, C.c_float, vp]
         L.fmvd_synth_blur.argtypes = [vp, vp, i, i, i, i, C.c_float, vp]
         L.fmvd_synth_poisson.argtypes = [vp, vp, C.c_long, C.c_ulonglong, i, vp]
+        L.fmvd_synth_copy.argtypes = [vp, vp, C.c_long, vp]
         _synth = L
     return _synth
 

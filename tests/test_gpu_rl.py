@@ -175,3 +175,8 @@ def test_invalid_device_reports_error(fm):
     assert errs and "HIP error" in errs[0]
     fm.lib().setCudaDevice(0)
     fm.check()
+    # the reported error must not resurface in the next kernel launch check
+    with fm.Deconvolver(16, 16, [np.ones((16, 16), np.float32)], [np.ones((3, 3), np.float32)]) as dec:
+        out, _ = dec.run([[np.full((16, 16), 7, np.uint16)]], 1)
+    fm.check()
+    assert out.shape == (1, 16, 16)
