@@ -439,3 +439,99 @@ def device_memory():
     lib().fmvd_device_memory(C.byref(f), C.byref(t))
     check()
     return f.value, t.value
+
+
+class BatchVolume(C.Structure):
+    """struct fmvd_batch_volume (fmvd_batch.h)."""
+    _fields_ = [("dataH", C.c_int), ("dataW", C.c_int), ("nplanes", C.c_int), ("nViews", C.c_int),
+                ("kernelH", C.c_int), ("kernelW", C.c_int), ("psf_type", C.c_int),
+                ("h_weights", C.POINTER(C.POINTER(C.c_float))), ("h_kernels", C.POINTER(C.POINTER(C.c_float))),
+                ("d_views", C.POINTER(C.c_void_p)), ("d_out", C.c_void_p), ("d_est", C.c_void_p)]
+
+
+class Batch:
+    """Mirror of the batched API (fmvd_batch.h): many independent volumes of
+    uneven size, one graph launch per RL iteration.
+
+    volumes: sequence of dicts with keys
+      views   -- list of V DeviceArrays (Z, H, W) of samples
+      weights -- list of V float32 arrays (H, W)
+      kernels -- list of V float32 arrays (kh, kw), normalised IN PLACE
+      psf_type (default 0), out (DeviceArray or None), est (DeviceArray or None)
+    """
+
+    def __init__(self, volumes, bits: int = 16):
+        L = lib()
+        if bits not in (8, 16):
+            raise ValueError("bits must be 8 or 16")
+        sfx = "16" if bits == 16 else "8"
+        self._create = getattr(L, "fmvd_batch_create_cuda_" + sfx)
+        self._run = getattr(L, "fmvd_batch_run_cuda_" + sfx)
+        self._tune = getattr(L, "fmvd_batch_autotune_cuda_" + sfx)
+        self._info = getattr(L, "fmvd_batch_info_cuda_" + sfx)
+        self._destroy = getattr(L, "fmvd_batch_destroy_cuda_" + sfx)
+        self._create.restype = C.c_void_p
+        self._create.argtypes = [C.c_int, C.POINTER(BatchVolume)]
+        self._run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        self._tune.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        self._info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        self._destroy.argtypes = [C.c_void_p]
+        self._keep = []
+        arr = (BatchVolume * len(volumes))()
+        for i, v in enumerate(volumes):
+            ks = [np.ascontiguousarray(k, dtype=np.float32) for k in v["kernels"]]
+            ws = [np.ascontiguousarray(w, dtype=np.float32) for w in v["weights"]]
+            nv = len(ks)
+            z, h, w = v["views"][0].shape
+            dv = (C.c_void_p * nv)(*[C.c_void_p(d.ptr) for d in v["views"]])
+            kp, wp = _float_ptr_array(ks), _float_ptr_array(ws)
+            self._keep += [ks, ws, dv, kp, wp]
+            arr[i] = BatchVolume(h, w, z, nv, ks[0].shape[0], ks[0].shape[1], int(v.get("psf_type", 0)), wp, kp, dv,
+                                 C.c_void_p(v["out"].ptr if v.get("out") is not None else None),
+                                 C.c_void_p(v["est"].ptr if v.get("est") is not None else None))
+            v["kernels_normalised"] = ks
+        self._keep.append(arr)
+        take_errors()
+        self.handle = self._create(len(volumes), arr)
+        check()
+        if not self.handle:
+            raise FmvdError("batch creation failed")
+
+    def info(self):
+        n = C.c_int()
+        self._info(self.handle, C.byref(n), None, None)
+        ext = (C.c_int * (2 * n.value))()
+        pl = (C.c_int * n.value)()
+        self._info(self.handle, C.byref(n), ext, pl)
+        check()
+        return [dict(fft_h=ext[2 * c], fft_w=ext[2 * c + 1], planes=pl[c]) for c in range(n.value)]
+
+    def autotune(self):
+        n = len(self.info())
+        chosen = (C.c_int * (3 * n))()
+        if self._tune(self.handle, chosen) != 0:
+            check()
+        check()
+        return [tuple(chosen[3 * c:3 * c + 3]) for c in range(n)]
+
+    def run(self, iterations: int, use_graph: bool = True, stream: int = 0) -> None:
+        if self._run(self.handle, int(iterations), 1 if use_graph else 0, C.c_void_p(stream or None)) != 0:
+            check()
+        check()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -267,9 +267,11 @@ template <int... Rs> struct RadixList {
     }
 };
 
-template <int N_, int T_, typename Radices> struct PlanDef {
+// G_ (optional): rows per workgroup of the row kernels (0 = their default).
+template <int N_, int T_, typename Radices, int G_ = 0> struct PlanDef {
     static constexpr int N = N_;
     static constexpr int T = T_;
+    static constexpr int G = G_;
     static constexpr int P = N_ / T_;
     using R = Radices;
     static constexpr int NP = Radices::count;
@@ -303,15 +305,16 @@ template <int N, int V> struct PlanV : Plan<N> {
     static constexpr bool distinct = (V == 0);
     static constexpr int variant = 0;
 };
-#define FMVD_PLAN_VARIANT(n, v, t, ...)                                          \
-    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>> {     \
+#define FMVD_PLAN_VARIANT_G(n, v, t, g, ...)                                     \
+    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>, g> {  \
         static constexpr bool distinct = true;                                   \
         static constexpr int variant = v;                                        \
     };
+#define FMVD_PLAN_VARIANT(n, v, t, ...) FMVD_PLAN_VARIANT_G(n, v, t, 0, __VA_ARGS__)
 FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
 FMVD_PLAN_VARIANT(512, 1, 64, 8, 8, 8)
-FMVD_PLAN_VARIANT(768, 1, 64, 4, 4, 4, 12)
-FMVD_PLAN_VARIANT(768, 2, 64, 4, 4, 4, 4, 3)
+FMVD_PLAN_VARIANT_G(768, 1, 64, 4, 4, 4, 4, 12)
+FMVD_PLAN_VARIANT_G(768, 2, 64, 8, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
 FMVD_PLAN_VARIANT(1280, 1, 32, 8, 8, 20)
@@ -320,6 +323,7 @@ FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
 FMVD_PLAN_VARIANT(1536, 2, 64, 4, 4, 8, 12)
 FMVD_PLAN_VARIANT(2560, 1, 64, 4, 8, 8, 10)
 #undef FMVD_PLAN_VARIANT
+#undef FMVD_PLAN_VARIANT_G
 
 // A plan is valid when its radices multiply to N and each divides P.
 template <typename PL> constexpr bool plan_valid()

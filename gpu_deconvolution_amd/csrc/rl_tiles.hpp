@@ -1,0 +1,427 @@
+// Tile-level device code of the RL passes, shared by the single-volume
+// kernels (rl_row_kernels.hip, rl_col_kernels.hip) and the batched kernels
+// over many volumes (rl_batch_kernels.hip).
+#pragma once
+#include "fft_device.hpp"
+#include "rl_launch.hpp"
+
+namespace fmvd {
+
+__device__ __forceinline__ int pad_index(int i, int n, int half)
+{
+    return i < n ? i : (i < n + half ? n - 1 : 0);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical block ids so neighbouring tiles share its L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblocks)
+{
+    const unsigned q = nblocks >> 3, rem = nblocks & 7u;
+    const unsigned x = bid & 7u, k = bid >> 3;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+// LDS row pitch >= len with 2*L = 64/G (mod 64) (4-byte banks): the
+// transposed tile access (G consecutive rows per 8-byte column element) then
+// touches distinct bank pairs across a half-wavefront.
+constexpr int row_pitch(int len, int G)
+{
+    const int want = (32 / G) % 32;
+    int L = len;
+    while (L % 32 != want) ++L;
+    return L;
+}
+
+template <typename PL_> struct RowCfg {
+    using PL = PL_;
+    static_assert(plan_valid<PL>(), "invalid FFT plan");
+    static constexpr int N = PL::N;
+    static constexpr int T = PL::T;
+    static constexpr int P = PL::P;
+    // rows per workgroup: the plan's choice, else 256 threads where possible
+    // with at least 8 rows so the transposed loads/stores move 64-byte runs
+    static constexpr int G = PL::G ? PL::G : ((256 / T) > 16 ? 16 : ((256 / T) < 8 ? 8 : (256 / T)));
+    static constexpr int L = row_pitch(lds_len<N>(), G);
+    static constexpr int THREADS = G * T;
+    static constexpr int RFI = first_radix<PL, +1>();  // layout on entry of the inverse
+    static constexpr int RLI = last_radix<PL, +1>();   // layout on exit of the inverse (= forward entry)
+    static constexpr int RLF = last_radix<PL, -1>();   // layout on exit of the forward
+    static_assert(RLI == first_radix<PL, -1>(), "inverse exit layout must match forward entry");
+};
+
+// LDS addressing.  lds_pad(base + c) = lds_pad(base) + c + c/16 whenever c is a
+// multiple of 16, and lds_pad(A - t) = A + A/16 + lds_mirror(t) whenever A is;
+// the affine forms let every access of a thread share one address register
+// and use immediate offsets.
+__device__ __forceinline__ int lds_mirror(int t) { return -t + ((-t) >> 4); }
+template <int C> constexpr int pad_c() { return C + C / 16; }
+
+// The affine forms apply when the sequence's thread count and every slot
+// offset of the given register layout are multiples of 16.
+template <typename C, int R> constexpr bool affine_layout()
+{
+    return C::T % 16 == 0 && (C::N / R) % 16 == 0 && C::N % 16 == 0;
+}
+
+// Transposed tile load: lds[g][kx] <- X[kx][y0+g].  Thread tid owns
+// g = tid % G and kx = tid / G + k*T (k < P).
+template <typename C>
+__device__ __forceinline__ void load_tile(float2* lds, const float2* __restrict__ X, int H, int y0, int tid)
+{
+    const int g = tid % C::G, kx0 = tid / C::G;
+    if constexpr (C::T % 16 == 0) {
+        const unsigned off0 = (unsigned)(kx0 * H + y0 + g), step = (unsigned)(C::T * H);
+        float2 tmp[C::P];
+#pragma unroll
+        for (int k = 0; k < C::P; ++k) tmp[k] = X[off0 + k * step];
+        float2* dst = lds + g * C::L + lds_pad(kx0);
+#pragma unroll
+        for (int k = 0; k < C::P; ++k) dst[k * pad_c<C::T>()] = tmp[k];
+    } else {
+#pragma unroll 4
+        for (int k = 0; k < C::P; ++k) {
+            const int kx = kx0 + k * C::T;
+            lds[g * C::L + lds_pad(kx)] = X[(unsigned)(kx * H + y0 + g)];
+        }
+    }
+}
+
+// C2R pre-processing: from the packed half-spectrum of one row (in LDS),
+// form the length-N complex input of the inverse FFT at this thread's
+// first-pass positions:  Z'[p] = A + iB,  A = X[p] + conj(X[N-p]),
+// B = (X[p] - conj(X[N-p])) exp(+2 pi i p / W);  Z'[0] = (DC+Ny, DC-Ny).
+__device__ __forceinline__ float2 c2r_combine(float2 xk, float2 xmc, float2 w)
+{
+    const float2 xm = cconj(xmc);
+    const float2 a = cadd(xk, xm);
+    const float2 b = cmulc(csub(xk, xm), w);
+    return make_float2(a.x - b.y, a.y + b.x);
+}
+
+template <typename C>
+__device__ __forceinline__ void c2r_pre(float2* v, const float2* row, int t, const float2* __restrict__ twW)
+{
+    using PL = typename C::PL;
+    if constexpr (affine_layout<C, C::RFI>()) {
+        const int pt = lds_pad(t), mt = lds_mirror(t);
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) {
+            constexpr int M = C::N / C::RFI;
+            const int c = (i / C::RFI) * C::T + (i % C::RFI) * M;  // slot_pos = t + c
+            const float2 xk = row[pt + c + c / 16];
+            if (i == 0) {
+                const int im = t == 0 ? pt : C::N + C::N / 16 + mt;
+                const float2 z = c2r_combine(xk, row[im], twW[t]);
+                v[i] = t == 0 ? make_float2(xk.x + xk.y, xk.x - xk.y) : z;
+            } else {
+                v[i] = c2r_combine(xk, row[(C::N - c) + (C::N - c) / 16 + mt], twW[t + c]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) {
+            const int p = slot_pos<PL, C::RFI>(t, i);
+            const float2 xk = row[lds_pad(p)];
+            if (p == 0) v[i] = make_float2(xk.x + xk.y, xk.x - xk.y);
+            else v[i] = c2r_combine(xk, row[lds_pad(C::N - p)], twW[p]);
+        }
+    }
+}
+
+// R2C post-processing and transposed store: v holds Z (forward FFT of the
+// packed row) at the forward's exit layout.  X[k] = (Z[k]+conj Z[N-k])/2 +
+// exp(-2 pi i k/W) (Z[k]-conj Z[N-k])/(2i);  X[0] = (ReZ0+ImZ0, ReZ0-ImZ0).
+__device__ __forceinline__ float2 r2c_combine(float2 zk, float2 zmc, float2 w)
+{
+    const float2 zm = cconj(zmc);
+    const float2 fe = cscale(cadd(zk, zm), 0.5f);
+    const float2 d = csub(zk, zm);
+    const float2 fo = make_float2(0.5f * d.y, -0.5f * d.x);
+    return cadd(fe, cmul(w, fo));
+}
+
+template <typename C>
+__device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, float2* __restrict__ X, int H,
+                                               int y0, int tid, int seq, int t, const float2* __restrict__ twW,
+                                               float scale)
+{
+    using PL = typename C::PL;
+    float2* row = lds + seq * C::L;
+    if constexpr (affine_layout<C, C::RLF>()) {
+        constexpr int M = C::N / C::RLF;
+        float2* dst = row + lds_pad(t);
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) {
+            const int c = (i / C::RLF) * C::T + (i % C::RLF) * M;
+            dst[c + c / 16] = v[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RLF>(t, i))] = v[i];
+    }
+    __syncthreads();
+    const int g = tid % C::G, kx0 = tid / C::G;
+    const float2* r = lds + g * C::L;
+    if constexpr (C::T % 16 == 0) {
+        const unsigned off0 = (unsigned)(kx0 * H + y0 + g), step = (unsigned)(C::T * H);
+        const int pk = lds_pad(kx0), mk = lds_mirror(kx0);
+#pragma unroll
+        for (int k = 0; k < C::P; ++k) {
+            const int c = k * C::T;
+            const float2 zk = r[pk + c + c / 16];
+            float2 xo;
+            if (k == 0) {
+                const int im = kx0 == 0 ? pk : C::N + C::N / 16 + mk;
+                const float2 z = r2c_combine(zk, r[im], twW[kx0]);
+                xo = kx0 == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y) : z;
+            } else {
+                xo = r2c_combine(zk, r[(C::N - c) + (C::N - c) / 16 + mk], twW[kx0 + c]);
+            }
+            X[off0 + k * step] = cscale(xo, scale);
+        }
+    } else {
+#pragma unroll 4
+        for (int k = 0; k < C::P; ++k) {
+            const int kx = kx0 + k * C::T;
+            const float2 zk = r[lds_pad(kx)];
+            const float2 xo = kx == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y)
+                                      : r2c_combine(zk, r[lds_pad(C::N - kx)], twW[kx]);
+            X[(unsigned)(kx * H + y0 + g)] = cscale(xo, scale);
+        }
+    }
+}
+
+// Reads the two values at x = 2n and 2n+1 (n = t + c) of one padded row
+// through the pad map (fmvd_deconvolve_cuda.impl.cu:34-50, pad_index).  c is
+// the register slot's compile-time offset, so every thread of a sequence
+// takes the same branch: all positions inside the data (one paired load when
+// `pairs`), all in the region that maps to x = 0, or the per-lane map.
+template <typename T, typename PairT>
+__device__ __forceinline__ void load_padded_pair(const T* __restrict__ row, int n, int c, int span, int dataW,
+                                                 int half, bool pairs, float& a, float& b)
+{
+    if (pairs && 2 * (c + span) <= dataW) {
+        const PairT w = reinterpret_cast<const PairT*>(row)[n];
+        if constexpr (sizeof(T) == 4) {
+            a = w.x;
+            b = w.y;
+        } else {
+            constexpr unsigned bits = 8 * sizeof(T), mask = (1u << bits) - 1u;
+            a = (float)(w & mask);
+            b = (float)((unsigned)w >> bits);
+        }
+    } else if (2 * c >= dataW + half) {
+        a = b = (float)row[0];
+    } else {
+        a = (float)row[pad_index(2 * n, dataW, half)];
+        b = (float)row[pad_index(2 * n + 1, dataW, half)];
+    }
+}
+template <typename T> struct PairOf;
+template <> struct PairOf<uint16_t> { using type = uint32_t; };
+template <> struct PairOf<uint8_t> { using type = uint16_t; };
+template <> struct PairOf<float> { using type = float2; };
+
+// Load a G-row tile and run the inverse real FFT of this thread's row:
+// on return v holds (f[2n], f[2n+1]) at the positions slot_pos<PL, RLI>.
+template <typename C>
+__device__ __forceinline__ void tile_c2r(float2* v, float2* lds, const float2* __restrict__ X, int H, int y0,
+                                         int tid, int seq, int t, const RowGeom& g)
+{
+    float2* row = lds + seq * C::L;
+    load_tile<C>(lds, X, H, y0, tid);
+    __syncthreads();
+    c2r_pre<C>(v, row, t, g.twW);
+    seq_sync<typename C::PL>();
+    fft_regs<typename C::PL, +1>(v, row, t, g.twP[C::PL::variant]);
+}
+
+
+// ------------------------------------------------------------- row tiles
+// Each works on the G rows [y0, y0+G) of one padded plane; lds holds
+// G*RowCfg::L complex.  g carries the FFT extent (uniform over the launch)
+// and the data extent and pad half-widths of the plane's volume.
+
+// Pass B: X <- R2C(Q > 1 ? D / Q : Q), Q = C2R(X).  divide_kernel_{16,8},
+// fmvd_deconvolve_cuda.impl.cu:159-176; the data is read unpadded through the
+// data pad map (swapped half-widths, fmvd_deconvolve.impl.cpp:36-37).
+template <typename SAMPLE, typename PL>
+__device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict__ Xp,
+                                           const SAMPLE* __restrict__ dplane, int y0, float2* lds)
+{
+    using C = RowCfg<PL>;
+    const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
+    const int y = y0 + seq;
+    float2* row = lds + seq * C::L;
+    float2 v[C::P];
+    tile_c2r<C>(v, lds, Xp, g.H, y0, tid, seq, t, g);
+
+    const SAMPLE* drow = dplane + (unsigned)(pad_index(y, g.dataH, g.dkY) * g.dataW);
+    const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)dplane % (2 * sizeof(SAMPLE)) == 0);
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
+        const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
+        const int n = t + c;
+        const float q0 = v[i].x, q1 = v[i].y;
+        float d0, d1;
+        load_padded_pair<SAMPLE, typename PairOf<SAMPLE>::type>(drow, n, c, C::T, g.dataW, g.dkX, pairs, d0, d1);
+        // hardware reciprocal (1 ulp) instead of the IEEE division sequence
+        v[i].x = q0 > 1.0f ? d0 * __builtin_amdgcn_rcpf(q0) : q0;
+        v[i].y = q1 > 1.0f ? d1 * __builtin_amdgcn_rcpf(q1) : q1;
+    }
+    fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
+    r2c_post_store<C>(v, lds, Xp, g.H, y0, tid, seq, t, g.twW, 1.0f);
+}
+
+// Pass D: C = C2R(XC); E <- E + w (E C - E) on the spatial estimate (one
+// fftH x fftW float plane, kept in the spatial domain as the reference does);
+// XE <- R2C(E) / W.  multiply32_kernel, fmvd_deconvolve_cuda.cu:374-392.
+template <typename PL>
+__device__ __forceinline__ void update_tile(const RowGeom& g, float2* __restrict__ XEp,
+                                            const float2* __restrict__ XCp, float* __restrict__ ESp,
+                                            const float* __restrict__ weights, int y0, float2* lds)
+{
+    using C = RowCfg<PL>;
+    const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
+    const int y = y0 + seq;
+    float2* row = lds + seq * C::L;
+    float2 v[C::P];
+    tile_c2r<C>(v, lds, XCp, g.H, y0, tid, seq, t, g);
+
+    float2* erow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
+    const float* wrow = weights + (unsigned)(pad_index(y, g.dataH, g.wkY) * g.dataW);
+    const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)weights % 8 == 0);
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
+        const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
+        const int n = t + c;
+        const float2 e = erow[n];
+        float w0, w1;
+        load_padded_pair<float, float2>(wrow, n, c, C::T, g.dataW, g.wkX, pairs, w0, w1);
+        // fmvd_deconvolve_cuda.cu:387-391 operation order
+        float t0 = e.x * v[i].x, t1 = e.y * v[i].y;
+        float c0 = t0 - e.x, c1 = t1 - e.y;
+        c0 *= w0;
+        c1 *= w1;
+        v[i].x = e.x + c0;
+        v[i].y = e.y + c1;
+        erow[n] = v[i];
+    }
+    fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
+    r2c_post_store<C>(v, lds, XEp, g.H, y0, tid, seq, t, g.twW, 1.0f / (float)g.W);
+}
+
+// Warm-start ingest: ES <- pad(E0) (data pad map), XE <- R2C(ES) / W.
+template <typename PL>
+__device__ __forceinline__ void init_tile(const RowGeom& g, float2* __restrict__ XEp, float* __restrict__ ESp,
+                                          const float* __restrict__ e0p, int y0, float2* lds)
+{
+    using C = RowCfg<PL>;
+    const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
+    const int y = y0 + seq;
+    float2* row = lds + seq * C::L;
+    const float* erow = e0p + (unsigned)(pad_index(y, g.dataH, g.dkY) * g.dataW);
+    float2* esrow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
+    float2 v[C::P];
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
+        const int n = slot_pos<PL, C::RLI>(t, i);
+        v[i].x = erow[pad_index(2 * n, g.dataW, g.dkX)];
+        v[i].y = erow[pad_index(2 * n + 1, g.dataW, g.dkX)];
+        esrow[n] = v[i];
+    }
+    fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
+    r2c_post_store<C>(v, lds, XEp, g.H, y0, tid, seq, t, g.twW, 1.0f / (float)g.W);
+}
+
+// ---------------------------------------------------------- column tile
+constexpr int pow2_floor(int x)
+{
+    int p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+
+template <typename PL_> struct ColCfg {
+    using PL = PL_;
+    static_assert(plan_valid<PL>(), "invalid FFT plan");
+    static constexpr int H = PL::N;
+    static constexpr int T = PL::T;
+    static constexpr int P = PL::P;
+    static constexpr int L = lds_len<H>() + 1;
+    // sequences per workgroup (a power of two, so it divides the row count):
+    // aim for 256 threads, keep LDS <= ~64 KB
+    static constexpr int SQ0 = (256 / T) < 1 ? 1 : (256 / T);
+    static constexpr int SQ1 = SQ0 > 16 ? 16 : SQ0;
+    static constexpr int SQ2 = 65536 / (L * 8) < 1 ? 1 : 65536 / (L * 8);
+    static constexpr int SQ = pow2_floor(SQ1 < SQ2 ? SQ1 : SQ2);
+    static constexpr int THREADS = SQ * T;
+    static constexpr int RF = first_radix<PL, -1>();
+    static constexpr int RL = last_radix<PL, -1>();
+    static_assert(RL == first_radix<PL, +1>(), "forward exit layout must match inverse entry");
+    static_assert(RF == last_radix<PL, +1>(), "inverse exit layout must match forward entry");
+};
+
+// Passes A / C on the kx-rows [grp*SQ, grp*SQ+SQ) of one plane:
+// Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]).  The cuFFT column halves plus
+// modulateAndNormalize of fmvd_deconvolve_cuda.cu:327-372 (normalisation
+// folded into K on the host).  kx = 0 holds two real columns packed as
+// DC + i*Nyquist: its spectrum is split with A[k] = (C[k] + conj C[-k])/2,
+// B[k] = (C[k] - conj C[-k])/(2i), multiplied by the DC and Nyquist kernel
+// columns and recombined as A*K0 + i*B*KN.
+template <typename PL>
+__device__ __forceinline__ void col_tile(const ColGeom& g, const float2* __restrict__ Xin, float2* __restrict__ Xout,
+                                         const float2* __restrict__ K, int grp, float2* lds)
+{
+    using C = ColCfg<PL>;
+    constexpr int H = C::H;
+    const int seq = threadIdx.x / C::T, t = threadIdx.x % C::T;
+    const int kx = grp * C::SQ + seq;
+    const float2* xin = Xin + (unsigned)(kx * H);
+    float2* row = lds + seq * C::L;
+
+    float2 v[C::P];
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) v[i] = xin[slot_pos<PL, C::RF>(t, i)];
+    fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
+
+    if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
+        if (seq == 0) {
+#pragma unroll
+            for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RL>(t, i))] = v[i];
+        }
+        seq_sync<PL>();
+        if (seq == 0) {
+            const float2* K0 = K;
+            const float2* KN = K + (unsigned)(g.N * H);
+#pragma unroll
+            for (int i = 0; i < C::P; ++i) {
+                const int k = slot_pos<PL, C::RL>(t, i);
+                const float2 ck = row[lds_pad(k)];
+                const float2 cm = cconj(row[lds_pad(k == 0 ? 0 : H - k)]);
+                const float2 a = cscale(cadd(ck, cm), 0.5f);
+                const float2 d = csub(ck, cm);
+                const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
+                const float2 ak = cmul(a, K0[k]);
+                const float2 bk = cmul(b, KN[k]);
+                v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
+            }
+        } else {
+            const float2* Kr = K + (unsigned)(kx * H);
+#pragma unroll
+            for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[slot_pos<PL, C::RL>(t, i)]);
+        }
+        seq_sync<PL>();
+    } else {
+        const float2* Kr = K + (unsigned)(kx * H);
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[slot_pos<PL, C::RL>(t, i)]);
+    }
+
+    fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);
+    float2* xo = Xout + (unsigned)(kx * H);
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) xo[slot_pos<PL, C::RF>(t, i)] = v[i];
+}
+
+}  // namespace fmvd

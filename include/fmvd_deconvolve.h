@@ -52,7 +52,9 @@ struct fmvd_plan_cuda_8;
 /* fmvd_deconvolve.impl.h:39-48 / fmvd_deconvolve.impl.cpp:130-203.
  * Normalises h_Kernel[v] in place (as the reference does), copies the
  * weights and retains the callbacks.  nstreams >= 1 is the minimum number of
- * planes kept in flight; the plan may batch more (see DESIGN). */
+ * planes kept in flight; the plan processes planes in batches of at least
+ * that many (fmvd_set_batch_planes_cuda_*), sized from a 2 GiB device budget
+ * by default. */
 struct fmvd_plan_cuda_16 *fmvd_initialize_cuda_16(
     int dataH, int dataW, float const *const *h_Weights, float **h_Kernel, int kernelH,
     int kernelW, fmvd_psf_type iteration_type, int nViews, int nstreams,

@@ -114,7 +114,8 @@ int orc_build_operands_f64(int dw, int dh, int nviews, const uint16_t *plane,
                            int kh, int psf_type, int ffth, int fftw,
                            double *Dp, double *Wp, double *kp, double *kph);
 
-/* fmvd_transform.impl.cu:24-56 with FP64 coordinates (see DESIGN):
+/* fmvd_transform.impl.cu:24-56 with FP64 coordinates and exact trilinear
+ * weights (the reference's texture unit uses 9-bit fixed-point weights):
  * out[z][y][x] for z in [z0, z0+nz). src is d x h x w. m = 12 floats,
  * row-major 3x4 target->source. */
 void orc_transform(const uint16_t *src, int w, int h, int d, int bits,

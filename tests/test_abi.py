@@ -107,3 +107,25 @@ def test_last_error_message(fm):
     buf = ctypes.create_string_buffer(256)
     n = fm.lib().fmvd_last_error(buf, 256)
     assert n > 0 and b"invalid transform dimensions" in buf.value
+
+
+def test_batch_rejects_bad_volumes_before_touching_the_device(fm):
+    """fmvd_batch_create validates every volume first (no GPU needed)."""
+    import ctypes as C
+    L = fm.lib()
+    L.fmvd_batch_create_cuda_16.restype = C.c_void_p
+    L.fmvd_batch_create_cuda_16.argtypes = [C.c_int, C.POINTER(fm.BatchVolume)]
+    w = np.ones((8, 8), np.float32)
+    k = np.ones((3, 3), np.float32)
+    wp, kp = fm._float_ptr_array([w]), fm._float_ptr_array([k])
+    dv = (C.c_void_p * 1)(C.c_void_p(0x1000))
+    vol = fm.BatchVolume(8, 8, 2, 1, 3, 3, 0, wp, kp, dv, None, None)   # FFT 16x16: unsupported
+    arr = (fm.BatchVolume * 1)(vol)
+    fm.take_errors()
+    assert not L.fmvd_batch_create_cuda_16(1, arr)
+    errs = fm.take_errors()
+    assert errs and "unsupported FFT extent" in errs[0]
+    arr[0].nViews = 0
+    assert not L.fmvd_batch_create_cuda_16(1, arr)
+    errs = fm.take_errors()
+    assert errs and "invalid volume dimensions" in errs[0]
