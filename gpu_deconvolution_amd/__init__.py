@@ -95,6 +95,8 @@ def lib() -> C.CDLL:
         "fmvd_autotune_cuda_8": (i, [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
         "fmvd_set_variants_cuda_16": (i, [vp, C.POINTER(C.c_int)]),
         "fmvd_set_variants_cuda_8": (i, [vp, C.POINTER(C.c_int)]),
+        "fmvd_convergence_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, C.POINTER(C.c_double), vp]),
+        "fmvd_convergence_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, C.POINTER(C.c_double), vp]),
         "fmvd_device_malloc": (vp, [C.c_size_t]),
         "fmvd_device_free": (None, [vp]),
         "fmvd_memcpy": (i, [vp, vp, C.c_size_t, i]),
@@ -269,6 +271,21 @@ class Deconvolver:
         check()
         if rc != 0:
             raise FmvdError("volume deconvolution failed")
+
+    def convergence(self, view_ptrs: Sequence[int], nplanes: int, iterations: int, ref_ptr: int,
+                    init_ptr: int = 0, stream: int = 0) -> np.ndarray:
+        """Relative L2 distance to the reference volume after ingest and after
+        every iteration (FP64 sums on the device): iterations + 1 values."""
+        L = lib()
+        fn = L.fmvd_convergence_cuda_16 if self.bits == 16 else L.fmvd_convergence_cuda_8
+        arr = (C.c_void_p * self.nviews)(*[C.c_void_p(int(p)) for p in view_ptrs])
+        errs = (C.c_double * (iterations + 1))()
+        rc = fn(self.plan, arr, int(nplanes), int(iterations), C.c_void_p(init_ptr or None), C.c_void_p(ref_ptr),
+                errs, C.c_void_p(stream or None))
+        check()
+        if rc != 0:
+            raise FmvdError("convergence run failed")
+        return np.array(errs[:])
 
     def close(self):
         if getattr(self, "plan", None):
@@ -535,3 +552,102 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Net:
+    """Mirror of the warm-start network API (fmvd_net.h)."""
+
+    def __init__(self, seed: int = 0):
+        L = lib()
+        vp, i, f = C.c_void_p, C.c_int, C.c_float
+        sigs = {
+            "fmvd_net_param_count": (C.c_size_t, []),
+            "fmvd_net_create": (vp, [C.c_ulonglong]),
+            "fmvd_net_destroy": (None, [vp]),
+            "fmvd_net_get_params": (i, [vp, vp]),
+            "fmvd_net_set_params": (i, [vp, vp]),
+            "fmvd_net_get_grads": (i, [vp, vp]),
+            "fmvd_net_forward": (i, [vp, vp, i, i, i, i, vp, vp]),
+            "fmvd_net_backward": (i, [vp, vp, vp, i, i, i, i, C.POINTER(C.c_double), vp]),
+            "fmvd_net_adam": (i, [vp, f, vp]),
+            "fmvd_net_prep_input": (i, [vp, i, i, i, i, i, vp, vp, vp]),
+            "fmvd_net_infer": (i, [vp, vp, i, i, i, i, f, f, vp, vp]),
+            "fmvd_net_infer_tiled": (i, [vp, vp, i, i, i, i, f, f, i, i, i, i, vp, vp]),
+            "fmvd_net_mfma_layout_test": (i, [vp, vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        self.L = L
+        self.nparams = int(L.fmvd_net_param_count())
+        take_errors()
+        self.handle = L.fmvd_net_create(int(seed))
+        check()
+
+    def _rc(self, rc):
+        check()
+        if rc != 0:
+            raise FmvdError("network call failed")
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        self._rc(self.L.fmvd_net_get_params(self.handle, out.ctypes.data))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, np.float32)
+        assert a.size == self.nparams
+        self._rc(self.L.fmvd_net_set_params(self.handle, a.ctypes.data))
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        self._rc(self.L.fmvd_net_get_grads(self.handle, out.ctypes.data))
+        return out
+
+    def forward(self, d_x: "DeviceArray", shape, d_y: "DeviceArray", stream: int = 0) -> None:
+        N, D, H, W = shape
+        self._rc(self.L.fmvd_net_forward(self.handle, d_x.ptr, N, D, H, W, d_y.ptr, stream or None))
+
+    def backward(self, d_x: "DeviceArray", d_t: "DeviceArray", shape, stream: int = 0) -> float:
+        N, D, H, W = shape
+        loss = C.c_double()
+        self._rc(self.L.fmvd_net_backward(self.handle, d_x.ptr, d_t.ptr, N, D, H, W, C.byref(loss), stream or None))
+        return loss.value
+
+    def adam(self, lr: float, stream: int = 0) -> None:
+        self._rc(self.L.fmvd_net_adam(self.handle, float(lr), stream or None))
+
+    def infer(self, views, scale: float, d_out: "DeviceArray", floor_value: float = 1e-3, tile=None,
+              stream: int = 0) -> None:
+        """views: list of DeviceArrays (D, H, W) uint16; d_out float (D, H, W)."""
+        D, H, W = views[0].shape
+        ptrs = DeviceArray((len(views),), np.uint64)
+        ptrs.upload(np.array([v.ptr for v in views], np.uint64))
+        if tile is None:
+            rc = self.L.fmvd_net_infer(self.handle, ptrs.ptr, len(views), D, H, W, float(scale), float(floor_value),
+                                       d_out.ptr, stream or None)
+        else:
+            tz, ty, tx, halo = tile
+            rc = self.L.fmvd_net_infer_tiled(self.handle, ptrs.ptr, len(views), D, H, W, float(scale),
+                                             float(floor_value), tz, ty, tx, halo, d_out.ptr, stream or None)
+        self._rc(rc)
+        device_synchronize()
+        ptrs.free()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.L.fmvd_net_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_bf16_bits(a: np.ndarray) -> np.ndarray:
+    """float -> bfloat16 bit patterns (round to nearest even), uint16."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -170,8 +171,14 @@ template <typename S> struct Plan {
     }
 
     // RL on `np` resident planes; views[v] points at the first plane of view v.
+    // Optional convergence trace: when set, run_batch adds after ingest and
+    // after every iteration k the sums (E - ref)^2 and ref^2 of its planes
+    // into errAcc[2k], errAcc[2k + 1] (device FP64).
+    const float* errRef = nullptr;
+    double* errAcc = nullptr;
+
     void run_batch(const S* const* views, long dstride, int np, int iterations, const float* init,
-                   long istride, S* out, float* est, long ostride, hipStream_t s)
+                   long istride, S* out, float* est, long ostride, hipStream_t s, long refOffset = 0)
     {
         ensure_work(np);
         const RowGeom rg = row_geom();
@@ -186,6 +193,7 @@ template <typename S> struct Plan {
             if (init) launch_row_init(rg, xe, es, xs, esd, init, istride, np, var.ingest, s);
             else launch_const_init(rg, xe, es, xs, esd, 10.0f * (float)nViews, np, s);
         }
+        if (errAcc) launch_error_sums(rg, es, esd, errRef + refOffset, (long)plane(), np, errAcc, s);
         for (int it = 0; it < iterations; ++it) {
             for (int v = 0; v < nViews; ++v) {
                 {
@@ -205,6 +213,8 @@ template <typename S> struct Plan {
                     launch_row_update(rg, xe, xt, es, xs, esd, Wn[v].as<float>(), np, var.update, s);
                 }
             }
+            if (errAcc)
+                launch_error_sums(rg, es, esd, errRef + refOffset, (long)plane(), np, errAcc + 2 * (it + 1), s);
         }
         TimedScope ts(kTimeEgress, s);
         launch_final<S>(rg, es, esd, out, est, ostride, np, s);
@@ -293,8 +303,35 @@ template <typename S> struct Plan {
             for (int v = 0; v < nViews; ++v) vv[v] = dviews[v] + (size_t)start * pl;
             run_batch(vv.data(), (long)pl, n, iterations, init ? init + (size_t)start * pl : nullptr,
                       (long)pl, out ? out + (size_t)start * pl : nullptr,
-                      est ? est + (size_t)start * pl : nullptr, (long)pl, s);
+                      est ? est + (size_t)start * pl : nullptr, (long)pl, s, (long)start * (long)pl);
         }
+    }
+
+        // RL on a GPU-resident volume recording, after ingest and after every
+    // iteration, the relative L2 distance of the estimate (data region) to
+    // ref; errs receives iterations + 1 values.
+    void convergence(const S* const* dviews, int nplanes, int iterations, const float* init, const float* ref,
+                     double* errs, hipStream_t s)
+    {
+        if (!ref || !errs) throw_msg("convergence trace needs a reference volume and an output array");
+        DeviceBuffer acc;
+        acc.alloc(sizeof(double) * 2 * (size_t)(iterations + 1));
+        FMVD_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, s));
+        errRef = ref;
+        errAcc = acc.as<double>();
+        try {
+            run_volume(dviews, nplanes, iterations, init, nullptr, nullptr, s);
+        } catch (...) {
+            errRef = nullptr;
+            errAcc = nullptr;
+            throw;
+        }
+        errRef = nullptr;
+        errAcc = nullptr;
+        std::vector<double> h(2 * (size_t)(iterations + 1));
+        FMVD_HIP(hipMemcpyAsync(h.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, s));
+        FMVD_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k <= iterations; ++k) errs[k] = h[2 * k + 1] > 0 ? std::sqrt(h[2 * k] / h[2 * k + 1]) : 0.0;
     }
 
     // fmvd_deconvolve.impl.cpp:205-300, as a double-buffered batch pipeline:
@@ -547,6 +584,29 @@ int fmvd_deconvolve_volume_device_cuda_16(const struct fmvd_plan_cuda_16* plan,
         if (!plan) fmvd::throw_msg("NULL plan");
         const_cast<fmvd_plan_cuda_16*>(plan)->impl.run_volume(d_views, nPlanes, iterations, d_init, d_out,
                                                               d_estimate, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_convergence_cuda_16(const struct fmvd_plan_cuda_16* plan, const unsigned short* const* d_views,
+                             int nPlanes, int iterations, const float* d_init, const float* d_ref, double* errs,
+                             void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_16*>(plan)->impl.convergence(d_views, nPlanes, iterations, d_init, d_ref, errs,
+                                                               (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_convergence_cuda_8(const struct fmvd_plan_cuda_8* plan, const unsigned char* const* d_views, int nPlanes,
+                            int iterations, const float* d_init, const float* d_ref, double* errs, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_8*>(plan)->impl.convergence(d_views, nPlanes, iterations, d_init, d_ref, errs,
+                                                              (hipStream_t)stream);
         return 0;
     })
 }

@@ -60,6 +60,11 @@ void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xst
 std::vector<float2> row_pass_twiddles(int N, int variant);
 std::vector<float2> col_pass_twiddles(int H, int variant);
 
+// acc[0] += sum (ES_crop - ref)^2, acc[1] += sum ref^2 over nplanes planes
+// (FP64 accumulation on the device).
+void launch_error_sums(const RowGeom& g, const float* ES, long estride, const float* ref, long rstride, int nplanes,
+                       double* acc, hipStream_t s);
+
 bool row_size_supported(int N);
 bool col_size_supported(int H);
 // Number of compiled FFT-plan variants for a row / column length (>= 1 when

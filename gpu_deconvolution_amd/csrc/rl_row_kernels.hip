@@ -109,6 +109,45 @@ __global__ void const_init_kernel(float2* __restrict__ XE, float* __restrict__ E
     reinterpret_cast<float2*>(ES + p * estride)[i] = make_float2(value, value);
 }
 
+// ------------------------------------------------------- convergence sums
+__global__ void error_sums_kernel(const float* __restrict__ ES, long estride, int W, int dataH, int dataW,
+                                  const float* __restrict__ ref, long rstride, double* __restrict__ acc)
+{
+    __shared__ double se[256], sr[256];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, p = blockIdx.z;
+    double e = 0.0, r = 0.0;
+    if (x < dataW) {
+        const double rv = ref[p * rstride + (long)y * dataW + x];
+        const double d = (double)ES[p * estride + (long)y * W + x] - rv;
+        e = d * d;
+        r = rv * rv;
+    }
+    se[threadIdx.x] = e;
+    sr[threadIdx.x] = r;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            se[threadIdx.x] += se[threadIdx.x + k];
+            sr[threadIdx.x] += sr[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, se[0]);
+        atomicAdd(acc + 1, sr[0]);
+    }
+}
+
+void launch_error_sums(const RowGeom& g, const float* ES, long estride, const float* ref, long rstride, int nplanes,
+                       double* acc, hipStream_t s)
+{
+    if (nplanes <= 0) return;
+    dim3 grid((unsigned)((g.dataW + 255) / 256), (unsigned)g.dataH, (unsigned)nplanes);
+    error_sums_kernel<<<grid, 256, 0, s>>>(ES, estride, g.W, g.dataH, g.dataW, ref, rstride, acc);
+    FMVD_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------- dispatchers
 #define FMVD_ROW_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(768) X(1024) X(1280) X(1536) X(1792) X(2048)
 

@@ -153,6 +153,18 @@ void fmvd_kernel_timing_enable(int on);
 void fmvd_kernel_timing_reset(void);
 int fmvd_kernel_timing_read(int kind, double *total_ms, long long *launches);
 
+/* Extension (no reference counterpart): RL on a GPU-resident volume, as
+ * fmvd_deconvolve_volume_device_cuda_*, that records after ingest and after
+ * every iteration k the relative L2 distance ||E_k - ref|| / ||ref|| of the
+ * estimate over the data region to d_ref (float, nPlanes*dataH*dataW,
+ * device), summed in FP64 on the device.  errs receives iterations + 1
+ * values (errs[0] for the start estimate).  Synchronous. */
+int fmvd_convergence_cuda_16(const struct fmvd_plan_cuda_16 *plan, const unsigned short *const *d_views,
+                             int nPlanes, int iterations, const float *d_init, const float *d_ref, double *errs,
+                             void *stream);
+int fmvd_convergence_cuda_8(const struct fmvd_plan_cuda_8 *plan, const unsigned char *const *d_views, int nPlanes,
+                            int iterations, const float *d_init, const float *d_ref, double *errs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

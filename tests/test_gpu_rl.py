@@ -180,3 +180,22 @@ def test_invalid_device_reports_error(fm):
         out, _ = dec.run([[np.full((16, 16), 7, np.uint16)]], 1)
     fm.check()
     assert out.shape == (1, 16, 16)
+
+
+def test_convergence_trace_matches_oracle_estimates(fm, orc):
+    """errs[k] = ||E_k - ref|| / ||ref||, with ref the FP64 oracle's estimate
+    after 4 iterations: errs[4] is the FP32/FP64 gap, and errs[k] for k < 4
+    matches the distance between the oracle's own k- and 4-iteration estimates."""
+    Z, V, dh, dw, k = 3, 2, 90, 110, 15
+    data, kernels, weights = make_case(Z, V, dh, dw, k, k, 16, seed=21)
+    ests = [orc.rl_planes(data, weights, kernels.copy(), 0, it, threads=8)[1] for it in range(5)]
+    ref = ests[4].astype(np.float32)
+    dviews = [to_device(np.ascontiguousarray(data[:, v])) for v in range(V)]
+    dref = to_device(ref)
+    with fm.Deconvolver(dh, dw, list(weights), [x.copy() for x in kernels], batch_planes=2) as dec:
+        errs = dec.convergence([t.ptr for t in dviews], Z, 4, dref.ptr)
+    want = [np.linalg.norm(e - ests[4]) / np.linalg.norm(ests[4]) for e in ests]
+    print("\\n[convergence]", errs, want)
+    assert errs[4] < 1e-5
+    for a, b in zip(errs[:4], want[:4]):
+        assert abs(a - b) <= 1e-4 * b + 1e-6
