@@ -47,12 +47,16 @@ def parse():
     p.add_argument("--cpu-planes", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the batched uneven-volume measurement")
+    p.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    p.add_argument("--train-samples", type=int, default=2000)
+    p.add_argument("--epochs", type=int, default=3)
     p.add_argument("--traffic-run", action="store_true",
                    help="short run for HBM counter collection: 64 planes, 1 iteration, 1 step, no CPU "
                         "baseline, then a 1 GiB device-to-device copy as the counters' calibration kernel")
     args = p.parse_args()
     if args.traffic_run:
         args.planes, args.iterations, args.steps, args.warmup, args.no_cpu, args.no_batch = 64, 1, 1, 0, True, True
+        args.no_learned = True
     return args
 
 
@@ -110,6 +114,81 @@ def batched_workload(fm, synth, seed=2024, nvol=48, iters=25, kernel=31):
                         f"{kernel}x{kernel} PSFs, {iters} iterations as hipGraph replays",
             "value": vvi * iters / el, "unit": "view-voxel RL iterations/s", "seconds": el,
             "classes": classes, "fft_plan_variants": tuned}
+
+
+def learned_leg(fm, oracle, ntrain=2000, epochs=3, label_iters=50, ntest=2, lr=2e-4, seed=7, tol=1e-3):
+    """Learned warm start vs plain RL (the 128 x 128 x 64, 2-view training
+    workload): train the network on `ntrain` (views -> `label_iters`-iteration
+    plain RL) pairs, then on held-out seeds compare time to a `tol` relative-L2
+    stop against the FP64 oracle's `label_iters`-iteration RL (the judge).  The
+    warm start is rejected (plain RL is the fallback) when its trajectory does
+    not reach the stop within label_iters + 20 iterations."""
+    from gpu_deconvolution_amd import warmstart as ws
+    shape, V = (64, 128, 128), 2
+    Z, H, W = shape
+    t0 = time.time()
+    samples = ws.make_samples(ntrain + ntest, shape, V, 31, label_iters, seed=seed, log=log)
+    train, test = samples[:ntrain], samples[ntrain:]
+    gen_s = time.time() - t0
+    net = fm.Net(seed)
+    tr = ws.Trainer(net, shape, V, batch=8)
+    t0 = time.time()
+    hist = tr.fit(train, epochs, lr=lr, log=log, lr_decay=0.75)
+    fm.device_synchronize()
+    train_s = time.time() - t0
+    tr.free()
+    for s in train:
+        s.free()
+    results = []
+    for s in test:
+        data = np.stack([v.download() for v in s.views], 1)
+        t = time.perf_counter()
+        _, ref = oracle.rl_planes(data, np.stack(s.weights), np.stack(s.kernels).copy(), 0, label_iters,
+                                  threads=cpu_threads())
+        oracle_s = time.perf_counter() - t
+        dref = fm.DeviceArray.from_host(ref.astype(np.float32))
+        init = fm.DeviceArray(shape, np.float32)
+        ptrs = [v.ptr for v in s.views]
+        with fm.Deconvolver(H, W, s.weights, [k.copy() for k in s.kernels]) as dec:
+            ep = dec.convergence(ptrs, Z, label_iters + 20, dref.ptr)
+            fm.device_synchronize()
+            t = time.perf_counter()
+            ws.warm_start(net, s.views, s.scale, init)
+            fm.device_synchronize()
+            infer_s = time.perf_counter() - t
+            ew = dec.convergence(ptrs, Z, label_iters + 20, dref.ptr, init_ptr=init.ptr)
+            kp = next((k for k, e in enumerate(ep) if e <= tol), None)
+            kw = next((k for k, e in enumerate(ew) if e <= tol), None)
+
+            def timed(iters, init_ptr=0):
+                fm.device_synchronize()
+                t1 = time.perf_counter()
+                dec.run_device(ptrs, Z, iters, init_ptr=init_ptr)
+                fm.device_synchronize()
+                return time.perf_counter() - t1
+
+            timed(2)
+            plain_s = timed(kp) if kp is not None else None
+            warm_iter_s = timed(kw, init.ptr) if kw is not None else None
+        best = float(ew.min())
+        k_equiv = next((k for k, e in enumerate(ep) if e <= best), None)
+        accepted = kw is not None and plain_s is not None and infer_s + warm_iter_s < plain_s
+        learned_s = infer_s + warm_iter_s if accepted else infer_s + (plain_s or 0.0)
+        results.append(dict(k_plain=kp, k_warm=kw, accepted=accepted, plain_s=plain_s, learned_s=learned_s,
+                            infer_s=infer_s, speedup=(plain_s / learned_s) if plain_s and learned_s else None,
+                            net_rel_l2=float(ew[0]), best_warm_rel_l2=best, best_warm_k=int(ew.argmin()),
+                            plain_k_to_best=k_equiv, oracle_s=oracle_s))
+        log(f"[bench] learned: k_plain={kp} k_warm={kw} accepted={accepted} net relL2 {ew[0]:.3e} "
+            f"best warm {best:.3e}@{int(ew.argmin())} (plain reaches it at {k_equiv}); "
+            f"plain {plain_s} s, learned {learned_s:.4f} s")
+        for b in (dref, init):
+            b.free()
+        s.free()
+    return {"workload": f"learned warm start, {ntrain} training samples 128x128x64 x 2 views (labels: "
+                        f"{label_iters}-iteration plain RL), {epochs} epochs batch 8 Adam; held-out seeds run to a "
+                        f"{tol:g} relative-L2 stop vs the FP64 oracle's {label_iters}-iteration RL",
+            "generate_s": gen_s, "train_s": train_s, "loss_per_epoch": hist, "samples": results,
+            "speedup": float(np.mean([r["speedup"] for r in results])) if results else None}
 
 
 def cpu_threads():
@@ -246,6 +325,11 @@ def main():
         dec.close()
         batched = batched_workload(fm, synth)
 
+    learned = None
+    if not args.no_learned and not args.no_cpu:
+        import oracle
+        learned = learned_leg(fm, oracle, ntrain=args.train_samples, epochs=args.epochs)
+
     dk = kstats[dom]
     line = {
         "metric": METRIC,
@@ -282,6 +366,7 @@ def main():
         "transform_gbs": tr_gbs,
         "cpu_baseline": cpu,
         "batched": batched,
+        "learned_vs_plain": learned,
     }
     print(json.dumps(line), flush=True)
 

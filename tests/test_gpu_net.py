@@ -104,21 +104,24 @@ def test_adam_step_matches_oracle(fm):
 
 
 def test_training_reduces_loss(fm):
+    """80 Adam steps at lr 2e-4 on a smoothing task; the FP64 oracle trained
+    the same way falls to ~7% of its initial loss."""
     rng = np.random.default_rng(5)
     N, D, H, W = 2, 6, 16, 32
     x = onet.bf16(rng.uniform(0.2, 1.5, size=(N, D, H, W, 2)))
-    # a smooth target the network can fit: a local average of the input mean
     m = 0.5 * (x[..., 0] + x[..., 1])
-    t = (0.5 * m + 0.25 * (np.roll(m, 1, axis=3) + np.roll(m, -1, axis=3))).astype(np.float32)
+    mp = np.pad(m, ((0, 0), (0, 0), (0, 0), (1, 1)))
+    t = (0.5 * m + 0.25 * (mp[..., 2:] + mp[..., :-2])).astype(np.float32)
     net = fm.Net(3)
     dx = fm.DeviceArray.from_host(fm.to_bf16_bits(x))
     dt = fm.DeviceArray.from_host(t)
     losses = []
-    for _ in range(40):
+    for _ in range(80):
         losses.append(net.backward(dx, dt, t.shape))
-        net.adam(2e-3)
-    print("losses", losses[0], losses[-1])
+        net.adam(2e-4)
+    print("losses", [round(l, 5) for l in losses[::10]], losses[-1])
     assert losses[-1] < 0.5 * losses[0]
+    assert np.isfinite(losses).all()
 
 
 def test_tiled_inference_equals_whole_volume(fm):
