@@ -28,25 +28,12 @@ struct Mat12 {
     double m[12];
 };
 
-template <typename S>
-__device__ __forceinline__ double texel(const S* __restrict__ src, int w, int h, int d, int x, int y,
-                                        int z)
-{
-    if (x < 0 || y < 0 || z < 0 || x >= w || y >= h || z >= d) return 0.0;
-    return (double)src[((long)z * h + y) * w + x];
-}
-
-template <typename S>
-__global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ src, int w, int h, int d,
-                                                        S* __restrict__ dst, int tw, int th, int z0,
-                                                        Mat12 M)
+// One output sample at (x, y, z); fetch(ix, iy, iz) returns the texel or 0
+// outside the volume.  Fixed FP64 operation order, no contraction.
+template <typename S, typename Fetch>
+__device__ __forceinline__ S transform_sample(int w, int h, int d, const Mat12& M, int x, int y, int z, Fetch fetch)
 {
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int zl = blockIdx.z;
-    const int z = z0 + zl;
-    if (x >= tw || y >= th) return;
     const double xd = (double)x, yd = (double)y, zd = (double)z;
     const double rx = ((M.m[0] * xd + M.m[1] * yd) + M.m[2] * zd) + M.m[3];
     const double ry = ((M.m[4] * xd + M.m[5] * yd) + M.m[6] * zd) + M.m[7];
@@ -58,10 +45,10 @@ __global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ sr
         const double ax = rx - fx, ay = ry - fy, az = rz - fz;
         const double bx = 1.0 - ax, by = 1.0 - ay, bz = 1.0 - az;
         const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-        const double c00 = texel(src, w, h, d, ix, iy, iz) * bx + texel(src, w, h, d, ix + 1, iy, iz) * ax;
-        const double c10 = texel(src, w, h, d, ix, iy + 1, iz) * bx + texel(src, w, h, d, ix + 1, iy + 1, iz) * ax;
-        const double c01 = texel(src, w, h, d, ix, iy, iz + 1) * bx + texel(src, w, h, d, ix + 1, iy, iz + 1) * ax;
-        const double c11 = texel(src, w, h, d, ix, iy + 1, iz + 1) * bx + texel(src, w, h, d, ix + 1, iy + 1, iz + 1) * ax;
+        const double c00 = fetch(ix, iy, iz) * bx + fetch(ix + 1, iy, iz) * ax;
+        const double c10 = fetch(ix, iy + 1, iz) * bx + fetch(ix + 1, iy + 1, iz) * ax;
+        const double c01 = fetch(ix, iy, iz + 1) * bx + fetch(ix + 1, iy, iz + 1) * ax;
+        const double c11 = fetch(ix, iy + 1, iz + 1) * bx + fetch(ix + 1, iy + 1, iz + 1) * ax;
         const double c0 = c00 * by + c10 * ay;
         const double c1 = c01 * by + c11 * ay;
         val = c0 * bz + c1 * az;
@@ -69,11 +56,86 @@ __global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ sr
     constexpr double maxv = (double)((1u << (8 * sizeof(S))) - 1u);
     constexpr double scale = (double)(1u << (8 * sizeof(S)));
     const double yv = (val / maxv) * scale + 0.5;
-    S o;
-    if (yv >= maxv) o = (S)maxv;
-    else if (yv > 0.0) o = (S)floor(yv);
-    else o = (S)0;
-    dst[((long)zl * th + y) * tw + x] = o;
+    if (yv >= maxv) return (S)maxv;
+    if (yv > 0.0) return (S)floor(yv);
+    return (S)0;
+}
+
+// Output tile of a workgroup: 16 x 16 x 8 voxels, thread (tx, ty) computes
+// the 8 voxels of its column.  The tile's source footprint is the bounding
+// box of its 8 corner images (an affine map takes its extremes at corners),
+// widened by one texel for rounding and clamped to [-3, extent + 2], which
+// holds every texel an in-range sample can read.  When the box fits the
+// LDS brick it is staged with row-contiguous (coalesced) loads, zero outside
+// the volume, and all trilinear fetches come from LDS; otherwise the
+// workgroup gathers from global memory.
+constexpr int kTX = 16, kTY = 16, kTZ = 8;
+constexpr int kBrickBytes = 48 * 1024;
+
+template <typename S>
+__global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ src, int w, int h, int d,
+                                                        S* __restrict__ dst, int tw, int th, int z0, int nz,
+                                                        Mat12 M)
+{
+#pragma clang fp contract(off)
+    constexpr int CAP = kBrickBytes / (int)sizeof(S);
+    __shared__ S brick[CAP];
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, zt0 = z0 + blockIdx.z * kTZ;
+    const int x1 = min(x0 + kTX, tw) - 1, y1 = min(y0 + kTY, th) - 1, zt1 = min(zt0 + kTZ, z0 + nz) - 1;
+    // bounding box of the tile's source footprint (block-uniform)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int c = 0; c < 8; ++c) {
+        const double xd = (double)((c & 1) ? x1 : x0), yd = (double)((c & 2) ? y1 : y0),
+                     zd = (double)((c & 4) ? zt1 : zt0);
+        for (int a = 0; a < 3; ++a) {
+            const double r = ((M.m[4 * a] * xd + M.m[4 * a + 1] * yd) + M.m[4 * a + 2] * zd) + M.m[4 * a + 3];
+            lo[a] = fmin(lo[a], r);
+            hi[a] = fmax(hi[a], r);
+        }
+    }
+    const int ext[3] = {w, h, d};
+    int b0[3], bn[3];
+    bool outside = false;
+    long cells = 1;
+    for (int a = 0; a < 3; ++a) {
+        // (a margin keeps interior points that round past a corner value in the staged path; NaN -> outside)
+        if (!(hi[a] > -2.0 - 1e-6) || !(lo[a] < (double)ext[a] + 1.0 + 1e-6)) outside = true;
+        const double l = fmax(floor(lo[a]) - 1.0, -3.0), u = fmin(floor(hi[a]) + 2.0, (double)ext[a] + 2.0);
+        b0[a] = outside ? 0 : (int)l;
+        bn[a] = outside ? 1 : max(1, (int)u - (int)l + 1);
+        cells *= bn[a];
+    }
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x <= x1 && y <= y1;
+    if (outside) {  // every sample of the tile is outside the source: zeros
+        if (active)
+            for (int z = zt0; z <= zt1; ++z) dst[((long)(z - z0) * th + y) * tw + x] = (S)0;
+        return;
+    }
+    if (cells <= CAP) {
+        const int BX = bn[0], BY = bn[1];
+        for (int e = threadIdx.x; e < (int)cells; e += 256) {
+            const int gx = b0[0] + e % BX, gy = b0[1] + (e / BX) % BY, gz = b0[2] + e / (BX * BY);
+            const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < w && gy < h && gz < d;
+            brick[e] = in ? src[((long)gz * h + gy) * w + gx] : (S)0;
+        }
+        __syncthreads();
+        if (!active) return;
+        auto fetch = [&](int ix, int iy, int iz) -> double {
+            return (double)brick[((iz - b0[2]) * BY + (iy - b0[1])) * BX + (ix - b0[0])];
+        };
+        for (int z = zt0; z <= zt1; ++z)
+            dst[((long)(z - z0) * th + y) * tw + x] = transform_sample<S>(w, h, d, M, x, y, z, fetch);
+    } else {
+        if (!active) return;
+        auto fetch = [&](int ix, int iy, int iz) -> double {
+            if (ix < 0 || iy < 0 || iz < 0 || ix >= w || iy >= h || iz >= d) return 0.0;
+            return (double)src[((long)iz * h + iy) * w + ix];
+        };
+        for (int z = zt0; z <= zt1; ++z)
+            dst[((long)(z - z0) * th + y) * tw + x] = transform_sample<S>(w, h, d, M, x, y, z, fetch);
+    }
 }
 
 __global__ __launch_bounds__(256) void mask_kernel(float* __restrict__ out, int z, int w, int h, int d,
@@ -116,10 +178,9 @@ static void launch_transform(const S* src, int w, int h, int d, S* dst, int tw, 
                              const Mat12& M, hipStream_t s)
 {
     if (nz <= 0 || tw <= 0 || th <= 0) return;
-    dim3 block(64, 4);
-    dim3 grid((unsigned)((tw + 63) / 64), (unsigned)((th + 3) / 4), (unsigned)nz);
+    dim3 grid((unsigned)((tw + kTX - 1) / kTX), (unsigned)((th + kTY - 1) / kTY), (unsigned)((nz + kTZ - 1) / kTZ));
     TimedScope ts(kTimeTransform, s);
-    transform_kernel<S><<<grid, block, 0, s>>>(src, w, h, d, dst, tw, th, z0, M);
+    transform_kernel<S><<<grid, 256, 0, s>>>(src, w, h, d, dst, tw, th, z0, nz, M);
     FMVD_LAUNCH_CHECK();
 }
 
