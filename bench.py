@@ -275,18 +275,23 @@ def main():
     fh, fw = info["fft_h"], info["fft_w"]
     S, P = fh * (fw // 2 + 1), fh * fw
     per_plane = {"column": 16 * S, "ratio": 16 * S + 2 * P, "update": 16 * S + 8 * P}
-    planes_per_launch = min(Z, info["batch"])
+    # Each launch covers one cache-resident chunk of planes; the library times
+    # the launches of every 8th chunk (event pairs around every launch would
+    # add microseconds between kernels), all of them full chunks here.
+    planes_per_launch = min(Z, info["batch"], info["chunk"])
+    assert Z % planes_per_launch == 0, "timed chunks must be full"
     kstats = {}
     for k, b in per_plane.items():
         ms, n = timing[k]
         if n:
             avg = ms / n / 1000.0
-            kstats[k] = dict(total_ms=ms, launches=n, avg_ms=1000 * avg,
+            kstats[k] = dict(total_ms=ms, launches=n, avg_ms=1000 * avg, planes_per_launch=planes_per_launch,
                              achieved_gbs=b * planes_per_launch / avg / 1e9)
     dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
     B_RL = 64 * S + 10 * P
     rl_ms = sum(kstats[k]["total_ms"] for k in kstats)
-    iter_gbs = B_RL * V * ITERS * Z * args.steps / (rl_ms / 1000.0) / 1e9
+    # plane-view-iterations behind the timed launches: one ratio launch each
+    iter_gbs = B_RL * kstats["ratio"]["launches"] * planes_per_launch / (rl_ms / 1000.0) / 1e9
     tr_ms, tr_n = timing["transform"]
     tr_gbs = (2 * Z * H * W * 2) * tr_n / (tr_ms / 1000.0) / 1e9 if tr_n else None
     for k, s in kstats.items():

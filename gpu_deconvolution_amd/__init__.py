@@ -77,6 +77,8 @@ def lib() -> C.CDLL:
         "fmvd_set_estimate_sink_cuda_8": (i, [vp, ESINK_T, vp]),
         "fmvd_set_batch_planes_cuda_16": (i, [vp, i]),
         "fmvd_set_batch_planes_cuda_8": (i, [vp, i]),
+        "fmvd_set_chunk_planes_cuda_16": (i, [vp, i]),
+        "fmvd_set_chunk_planes_cuda_8": (i, [vp, i]),
         "fmvd_deconvolve_volume_device_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_deconvolve_volume_device_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_plan_info_cuda_16": (i, [vp, C.POINTER(C.c_int)]),
@@ -165,7 +167,7 @@ class Deconvolver:
 
     def __init__(self, data_h: int, data_w: int, weights: Sequence[np.ndarray],
                  kernels: Sequence[np.ndarray], psf_type: int = 0, bits: int = 16, nstreams: int = 3,
-                 batch_planes: int = 0):
+                 batch_planes: int = 0, chunk_planes: int = 0):
         L = lib()
         self.bits = bits
         self.h, self.w = int(data_h), int(data_w)
@@ -201,11 +203,13 @@ class Deconvolver:
             self._src, self._snk = SOURCE16_T(source), SINK16_T(sink)
             init, self._run, self._destroy = L.fmvd_initialize_cuda_16, L.fmvd_deconvolve_planes_cuda_16, L.fmvd_destroy_cuda_16
             self._set_esink, self._set_batch = L.fmvd_set_estimate_sink_cuda_16, L.fmvd_set_batch_planes_cuda_16
+            self._set_chunk = L.fmvd_set_chunk_planes_cuda_16
             self._volume, self._info = L.fmvd_deconvolve_volume_device_cuda_16, L.fmvd_plan_info_cuda_16
         elif bits == 8:
             self._src, self._snk = SOURCE8_T(source), SINK8_T(sink)
             init, self._run, self._destroy = L.fmvd_initialize_cuda_8, L.fmvd_deconvolve_planes_cuda_8, L.fmvd_destroy_cuda_8
             self._set_esink, self._set_batch = L.fmvd_set_estimate_sink_cuda_8, L.fmvd_set_batch_planes_cuda_8
+            self._set_chunk = L.fmvd_set_chunk_planes_cuda_8
             self._volume, self._info = L.fmvd_deconvolve_volume_device_cuda_8, L.fmvd_plan_info_cuda_8
         else:
             raise ValueError("bits must be 8 or 16")
@@ -218,14 +222,20 @@ class Deconvolver:
             raise FmvdError("plan creation failed")
         if batch_planes:
             self._set_batch(self.plan, int(batch_planes))
+        if chunk_planes:
+            self._set_chunk(self.plan, int(chunk_planes))
 
     def info(self) -> dict:
-        d = (C.c_int * 6)()
+        d = (C.c_int * 7)()
         self._info(self.plan, d)
-        return dict(data_h=d[0], data_w=d[1], fft_h=d[2], fft_w=d[3], views=d[4], batch=d[5])
+        return dict(data_h=d[0], data_w=d[1], fft_h=d[2], fft_w=d[3], views=d[4], batch=d[5], chunk=d[6])
 
     def set_batch_planes(self, n: int) -> int:
         return self._set_batch(self.plan, int(n))
+
+    def set_chunk_planes(self, n: int) -> int:
+        """Planes per cache-resident chunk (0 = automatic); returns the value in effect."""
+        return self._set_chunk(self.plan, int(n))
 
     KINDS = ("column", "ratio", "update", "ingest", "egress")
 

@@ -268,10 +268,14 @@ template <int... Rs> struct RadixList {
 };
 
 // G_ (optional): rows per workgroup of the row kernels (0 = their default).
-template <int N_, int T_, typename Radices, int G_ = 0> struct PlanDef {
+// W_ (optional): minimum resident waves per SIMD the kernels are compiled for
+// (amdgpu_waves_per_eu; 1 = whatever the workgroup size implies); a higher
+// value caps the VGPRs per thread so that more workgroups fit on a CU.
+template <int N_, int T_, typename Radices, int G_ = 0, int W_ = 1> struct PlanDef {
     static constexpr int N = N_;
     static constexpr int T = T_;
     static constexpr int G = G_;
+    static constexpr int WPE = W_;
     static constexpr int P = N_ / T_;
     using R = Radices;
     static constexpr int NP = Radices::count;
@@ -305,15 +309,16 @@ template <int N, int V> struct PlanV : Plan<N> {
     static constexpr bool distinct = (V == 0);
     static constexpr int variant = 0;
 };
-#define FMVD_PLAN_VARIANT_G(n, v, t, g, ...)                                     \
-    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>, g> {  \
-        static constexpr bool distinct = true;                                   \
-        static constexpr int variant = v;                                        \
+#define FMVD_PLAN_VARIANT_GW(n, v, t, g, w, ...)                                   \
+    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>, g, w> {  \
+        static constexpr bool distinct = true;                                     \
+        static constexpr int variant = v;                                          \
     };
+#define FMVD_PLAN_VARIANT_G(n, v, t, g, ...) FMVD_PLAN_VARIANT_GW(n, v, t, g, 1, __VA_ARGS__)
 #define FMVD_PLAN_VARIANT(n, v, t, ...) FMVD_PLAN_VARIANT_G(n, v, t, 0, __VA_ARGS__)
 FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
 FMVD_PLAN_VARIANT(512, 1, 64, 8, 8, 8)
-FMVD_PLAN_VARIANT_G(768, 1, 64, 4, 4, 4, 4, 12)
+FMVD_PLAN_VARIANT_GW(768, 1, 64, 8, 6, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT_G(768, 2, 64, 8, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
@@ -324,6 +329,7 @@ FMVD_PLAN_VARIANT(1536, 2, 64, 4, 4, 8, 12)
 FMVD_PLAN_VARIANT(2560, 1, 64, 4, 8, 8, 10)
 #undef FMVD_PLAN_VARIANT
 #undef FMVD_PLAN_VARIANT_G
+#undef FMVD_PLAN_VARIANT_GW
 
 // A plan is valid when its radices multiply to N and each divides P.
 template <typename PL> constexpr bool plan_valid()

@@ -97,7 +97,7 @@ void drain_locked()
 
 bool timing_on() { return g_timing; }
 
-TimedScope::TimedScope(int kind, hipStream_t s) : kind_(kind), s_(s), on_(g_timing)
+TimedScope::TimedScope(int kind, hipStream_t s, bool sample) : kind_(kind), s_(s), on_(g_timing && sample)
 {
     if (!on_) return;
     std::lock_guard<std::mutex> lk(g_tmu);

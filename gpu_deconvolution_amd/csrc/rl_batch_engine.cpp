@@ -47,7 +47,7 @@ template <typename S> struct Batch {
     struct Cls {
         int fftH = 0, fftW = 0, N = 0, nplanes = 0, maxV = 0, maxH = 0, maxW = 0;
         std::vector<int> vols;
-        DeviceBuffer XE, XT, ES, twW, twRow[3], twCol[3];
+        DeviceBuffer XE, ES, twW, twRow[3], twCol[3];  // XE: the in-place spectrum of every pass
         Variants var;
         Launch planes;
         std::vector<Launch> col, ratio, update;  // per view index
@@ -157,7 +157,6 @@ template <typename S> struct Batch {
                 upload(c.twCol[k], col_pass_twiddles(c.fftH, k));
             }
             c.XE.alloc(sizeof(float2) * (size_t)c.xs() * c.nplanes);
-            c.XT.alloc(sizeof(float2) * (size_t)c.xs() * c.nplanes);
             c.ES.alloc(sizeof(float) * (size_t)c.es() * c.nplanes);
         }
         // device segment table
@@ -272,7 +271,7 @@ template <typename S> struct Batch {
             const RowGeom rg = row_geom(c);
             const ColGeom cg = col_geom(c);
             float2* xe = c.XE.template as<float2>();
-            float2* xt = c.XT.template as<float2>();
+            float2* xt = xe;
             float* es = c.ES.template as<float>();
             for (int v = 0; v < c.maxV; ++v) {
                 launch_batch_col(cg, sg, L(c.col[v]), c.col[v].n, c.col[v].blocks, v, 0, xe, xt, c.xs(), c.var.column,
@@ -352,7 +351,7 @@ template <typename S> struct Batch {
                         const RowGeom rg = row_geom(c);
                         const ColGeom cg = col_geom(c);
                         float2* xe = c.XE.template as<float2>();
-                        float2* xt = c.XT.template as<float2>();
+                        float2* xt = xe;
                         float* es = c.ES.template as<float>();
                         if (kind == 0)
                             launch_batch_col(cg, sg, L(c.col[0]), c.col[0].n, c.col[0].blocks, 0, 0, xe, xt, c.xs(),

@@ -42,7 +42,7 @@ __device__ __forceinline__ RowGeom seg_geom(const RowGeom& g, const BatchSeg& S)
 }
 
 template <typename SAMPLE, typename PL>
-__global__ __launch_bounds__(RowCfg<PL>::THREADS) void batch_ratio_kernel(RowGeom g, const BatchSeg* __restrict__ segs,
+__global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void batch_ratio_kernel(RowGeom g, const BatchSeg* __restrict__ segs,
                                                                           const int* __restrict__ list, int n, int view,
                                                                           float2* __restrict__ XT, long xstride)
 {
@@ -58,10 +58,10 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS) void batch_ratio_kernel(RowGeo
 }
 
 template <typename PL>
-__global__ __launch_bounds__(RowCfg<PL>::THREADS) void batch_update_kernel(RowGeom g, const BatchSeg* __restrict__ segs,
+__global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void batch_update_kernel(RowGeom g, const BatchSeg* __restrict__ segs,
                                                                            const int* __restrict__ list, int n, int view,
-                                                                           float2* __restrict__ XE,
-                                                                           const float2* __restrict__ XT,
+                                                                           float2* XE,
+                                                                           const float2* XT,
                                                                            float* __restrict__ ES, long xstride,
                                                                            long estride)
 {
@@ -78,10 +78,10 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS) void batch_update_kernel(RowGe
 }
 
 template <typename PL>
-__global__ __launch_bounds__(ColCfg<PL>::THREADS) void batch_col_kernel(ColGeom g, const BatchSeg* __restrict__ segs,
+__global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void batch_col_kernel(ColGeom g, const BatchSeg* __restrict__ segs,
                                                                         const int* __restrict__ list, int n, int view,
-                                                                        int hat, const float2* __restrict__ Xin,
-                                                                        float2* __restrict__ Xout, long xstride)
+                                                                        int hat, const float2* Xin,
+                                                                        float2* Xout, long xstride)
 {
     using C = ColCfg<PL>;
     __shared__ float2 lds[C::SQ * C::L];

@@ -20,8 +20,8 @@
 namespace fmvd {
 
 template <typename PL>
-__global__ __launch_bounds__(ColCfg<PL>::THREADS) void col_conv_kernel(ColGeom g, const float2* __restrict__ Xin,
-                                                                       float2* __restrict__ Xout, long xstride,
+__global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(ColGeom g, const float2* Xin,
+                                                                       float2* Xout, long xstride,
                                                                        const float2* __restrict__ K, int nplanes)
 {
     using C = ColCfg<PL>;

@@ -4,14 +4,17 @@
 // Reference: fmvd_deconvolve.impl.cpp:42-336 (prepareKernels, prepareWeights,
 // fmvd_initialize_cuda, fmvd_deconvolve_planes_cuda, fmvd_destroy_cuda).
 //
-// Per view and iteration the device runs four launches over a whole batch of
-// planes (the reference runs eight per plane):
-//   A  column pass   XT = colconv(XE, K_v / H)                (spectrum of Q)
-//   B  row pass      XT = R2C(Q > 1 ? D_v / Q : Q), Q = C2R(XT)
-//   C  column pass   XT = colconv(XT, Khat_v / (H W))          (spectrum of C)
-//   D  row pass      C = C2R(XT), E += w_v (E C - E), XE = R2C(E)/W
+// Per view and iteration the device runs four launches over a chunk of
+// planes (the reference runs eight per plane), all in place on one spectrum
+// buffer X per plane:
+//   A  column pass   X = colconv(X, K_v / H)             X: XE -> spectrum of Q
+//   B  row pass      X = R2C(Q > 1 ? D_v / Q : Q), Q = C2R(X)
+//   C  column pass   X = colconv(X, Khat_v / (H W))      X: spectrum of C
+//   D  row pass      C = C2R(X), E += w_v (E C - E), X = XE = R2C(E)/W
 // E (ES) is the spatial estimate on the padded plane, as in the reference; XE
 // is its row spectrum scaled by 1/W, kept so that pass A needs no row FFT.
+// XE is dead from pass A's read to pass D's write, so it shares X with the
+// intermediate spectra.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,12 +56,12 @@ template <typename S> struct Plan {
     sink_t sink = nullptr;
     fmvd_estimate_sink_t esink = nullptr;
     void* euser = nullptr;
-    int batchCap = 0;
+    int batchCap = 0, chunkCap = 0;
     Variants var;
 
     DeviceBuffer twW, twRow[3], twCol[3];
     std::vector<DeviceBuffer> K, Kh, Wn;
-    DeviceBuffer XE, XT, ES;
+    DeviceBuffer XE, ES;  // XE: the spectrum buffer X of every pass
     std::unique_ptr<Stream> compute, copy;
 
     size_t plane() const { return (size_t)dataH * dataW; }
@@ -97,7 +100,7 @@ template <typename S> struct Plan {
     {
         if (batchCap > 0) return batchCap;
         const size_t budget = env_bytes("FMVD_BATCH_BYTES", (size_t)2 << 30);
-        const size_t per = 2 * xplane() * sizeof(float2) + eplane() * sizeof(float);
+        const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float);
         size_t nb = budget / (per ? per : 1);
         nb = std::max<size_t>(1, std::min<size_t>(nb, 4096));
         return (int)std::max<size_t>(nb, (size_t)std::max(1, nStreams));
@@ -106,7 +109,6 @@ template <typename S> struct Plan {
     void ensure_work(int planes)
     {
         XE.alloc(xplane() * sizeof(float2) * (size_t)planes);
-        XT.alloc(xplane() * sizeof(float2) * (size_t)planes);
         ES.alloc(eplane() * sizeof(float) * (size_t)planes);
     }
 
@@ -177,46 +179,80 @@ template <typename S> struct Plan {
     const float* errRef = nullptr;
     double* errAcc = nullptr;
 
+    // Cache-resident chunks.  Every plane's RL is independent, so run_batch
+    // splits its planes into chunks and runs each chunk's whole iteration
+    // chain before the next.  A chunk's X and ES (19 MB per plane at FFT
+    // 1536^2), its views' samples and one view's kernel spectra and weights
+    // then fit the 256 MB memory-side cache, so each pass finds most of its
+    // operands there instead of in HBM.  The default budget gives 8 planes
+    // per chunk at FFT 1536^2: 1536 row tiles and 1536 column groups per
+    // launch, whole multiples of the resident workgroups.
+    int chunk_planes() const
+    {
+        if (chunkCap > 0) return chunkCap;
+        const size_t budget = env_bytes("FMVD_CHUNK_BYTES", (size_t)192 << 20);
+        const size_t shared = 2 * xplane() * sizeof(float2) + plane() * sizeof(float);  // K_v, Khat_v, w_v
+        const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float) + plane() * sizeof(S);
+        const size_t room = budget > shared ? budget - shared : per;
+        return (int)std::max<size_t>(1, std::min<size_t>(room / per, 4096));
+    }
+
     void run_batch(const S* const* views, long dstride, int np, int iterations, const float* init,
                    long istride, S* out, float* est, long ostride, hipStream_t s, long refOffset = 0)
     {
         ensure_work(np);
+        const int c = std::max(1, std::min(np, chunk_planes()));
+        for (int start = 0, i = 0; start < np; start += c, ++i)
+            run_chunk(views, dstride, start, std::min(c, np - start), iterations, init, istride, out, est, ostride,
+                      s, refOffset, i % kTimingStride == 0);
+    }
+    // With kernel timing on, every kTimingStride-th chunk's launches are timed.
+    static constexpr int kTimingStride = 8;
+
+    // The iteration chain on planes [start, start + np) of a batch.
+    void run_chunk(const S* const* views, long dstride, int start, int np, int iterations, const float* init,
+                   long istride, S* out, float* est, long ostride, hipStream_t s, long refOffset, bool timed)
+    {
         const RowGeom rg = row_geom();
         const ColGeom cg = col_geom();
         const long xs = (long)xplane();
-        float2* xe = XE.as<float2>();
-        float2* xt = XT.as<float2>();
-        float* es = ES.as<float>();
         const long esd = (long)eplane();
+        float2* xe = XE.as<float2>() + (size_t)start * xs;
+        float2* xt = xe;  // passes A-C write the spectrum in place
+        float* es = ES.as<float>() + (size_t)start * esd;
+        const float* ref = errRef ? errRef + refOffset + (size_t)start * plane() : nullptr;
+        if (init) init += (size_t)start * istride;
+        if (out) out += (size_t)start * ostride;
+        if (est) est += (size_t)start * ostride;
         {
-            TimedScope ts(kTimeIngest, s);
+            TimedScope ts(kTimeIngest, s, timed);
             if (init) launch_row_init(rg, xe, es, xs, esd, init, istride, np, var.ingest, s);
             else launch_const_init(rg, xe, es, xs, esd, 10.0f * (float)nViews, np, s);
         }
-        if (errAcc) launch_error_sums(rg, es, esd, errRef + refOffset, (long)plane(), np, errAcc, s);
+        if (errAcc) launch_error_sums(rg, es, esd, ref, (long)plane(), np, errAcc, s);
         for (int it = 0; it < iterations; ++it) {
             for (int v = 0; v < nViews; ++v) {
+                const S* dv = views[v] + (size_t)start * dstride;
                 {
-                    TimedScope ts(kTimeColumn, s);
+                    TimedScope ts(kTimeColumn, s, timed);
                     launch_col_conv(cg, xe, xt, xs, K[v].as<float2>(), np, var.column, s);
                 }
                 {
-                    TimedScope ts(kTimeRatio, s);
-                    launch_row_ratio<S>(rg, xt, xs, views[v], dstride, np, var.ratio, s);
+                    TimedScope ts(kTimeRatio, s, timed);
+                    launch_row_ratio<S>(rg, xt, xs, dv, dstride, np, var.ratio, s);
                 }
                 {
-                    TimedScope ts(kTimeColumn, s);
+                    TimedScope ts(kTimeColumn, s, timed);
                     launch_col_conv(cg, xt, xt, xs, Kh[v].as<float2>(), np, var.column, s);
                 }
                 {
-                    TimedScope ts(kTimeUpdate, s);
+                    TimedScope ts(kTimeUpdate, s, timed);
                     launch_row_update(rg, xe, xt, es, xs, esd, Wn[v].as<float>(), np, var.update, s);
                 }
             }
-            if (errAcc)
-                launch_error_sums(rg, es, esd, errRef + refOffset, (long)plane(), np, errAcc + 2 * (it + 1), s);
+            if (errAcc) launch_error_sums(rg, es, esd, ref, (long)plane(), np, errAcc + 2 * (it + 1), s);
         }
-        TimedScope ts(kTimeEgress, s);
+        TimedScope ts(kTimeEgress, s, timed);
         launch_final<S>(rg, es, esd, out, est, ostride, np, s);
     }
 
@@ -227,7 +263,7 @@ template <typename S> struct Plan {
     void autotune(int planes, float* times)
     {
         FMVD_HIP(hipSetDevice(device));
-        planes = std::max(1, std::min(planes > 0 ? planes : 32, batch_planes()));
+        planes = std::max(1, std::min({planes > 0 ? planes : 32, batch_planes(), chunk_planes()}));
         ensure_work(planes);
         const size_t pl = plane();
         DeviceBuffer data, out, e0;
@@ -241,7 +277,7 @@ template <typename S> struct Plan {
         const ColGeom cg = col_geom();
         const long xs = (long)xplane();
         float2* xe = XE.as<float2>();
-        float2* xt = XT.as<float2>();
+        float2* xt = xe;
         float* es = ES.as<float>();
         const long esd = (long)eplane();
         launch_const_init(rg, xe, es, xs, esd, 10.0f * (float)nViews, planes, s);
@@ -456,6 +492,7 @@ template <typename PlanT> int plan_info(const PlanT* p, int* dims)
     dims[3] = p->impl.fftW;
     dims[4] = p->impl.nViews;
     dims[5] = p->impl.batch_planes();
+    dims[6] = p->impl.chunk_planes();
     return 0;
 }
 
@@ -573,6 +610,20 @@ int fmvd_set_batch_planes_cuda_8(struct fmvd_plan_cuda_8* plan, int planes)
     if (!plan) return -1;
     plan->impl.batchCap = planes > 0 ? planes : 0;
     return plan->impl.batch_planes();
+}
+
+int fmvd_set_chunk_planes_cuda_16(struct fmvd_plan_cuda_16* plan, int planes)
+{
+    if (!plan) return -1;
+    plan->impl.chunkCap = planes > 0 ? planes : 0;
+    return plan->impl.chunk_planes();
+}
+
+int fmvd_set_chunk_planes_cuda_8(struct fmvd_plan_cuda_8* plan, int planes)
+{
+    if (!plan) return -1;
+    plan->impl.chunkCap = planes > 0 ? planes : 0;
+    return plan->impl.chunk_planes();
 }
 
 int fmvd_deconvolve_volume_device_cuda_16(const struct fmvd_plan_cuda_16* plan,

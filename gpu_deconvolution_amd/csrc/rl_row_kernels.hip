@@ -22,7 +22,7 @@ namespace fmvd {
 
 // ------------------------------------------------------------------ pass B
 template <typename SAMPLE, typename PL>
-__global__ __launch_bounds__(RowCfg<PL>::THREADS) void row_ratio_kernel(
+__global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_ratio_kernel(
     RowGeom g, float2* __restrict__ X, long xstride, const SAMPLE* __restrict__ data, long dstride, int nplanes)
 {
     using C = RowCfg<PL>;
@@ -50,8 +50,8 @@ __device__ __forceinline__ void grouped_tile(unsigned lid, int nrg, int nplanes,
 }
 
 template <typename PL>
-__global__ __launch_bounds__(RowCfg<PL>::THREADS) void row_update_kernel(
-    RowGeom g, float2* __restrict__ XE, const float2* __restrict__ XC, float* __restrict__ ES, long xstride,
+__global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_update_kernel(
+    RowGeom g, float2* XE, const float2* XC, float* __restrict__ ES, long xstride,
     long estride, const float* __restrict__ weights, int nplanes)
 {
     using C = RowCfg<PL>;
@@ -86,7 +86,7 @@ __global__ void final_kernel(const float* __restrict__ ES, long estride, int W, 
 
 // ------------------------------------------------------------------ ingest
 template <typename PL>
-__global__ __launch_bounds__(RowCfg<PL>::THREADS) void row_init_kernel(
+__global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_init_kernel(
     RowGeom g, float2* __restrict__ XE, float* __restrict__ ES, long xstride, long estride,
     const float* __restrict__ e0, long e0stride, int nplanes)
 {

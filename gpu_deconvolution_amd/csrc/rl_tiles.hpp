@@ -276,9 +276,10 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
 // Pass D: C = C2R(XC); E <- E + w (E C - E) on the spatial estimate (one
 // fftH x fftW float plane, kept in the spatial domain as the reference does);
 // XE <- R2C(E) / W.  multiply32_kernel, fmvd_deconvolve_cuda.cu:374-392.
+// XE may be XC (in place): the tile is in LDS before any of it is written.
 template <typename PL>
-__device__ __forceinline__ void update_tile(const RowGeom& g, float2* __restrict__ XEp,
-                                            const float2* __restrict__ XCp, float* __restrict__ ESp,
+__device__ __forceinline__ void update_tile(const RowGeom& g, float2* XEp,
+                                            const float2* XCp, float* __restrict__ ESp,
                                             const float* __restrict__ weights, int y0, float2* lds)
 {
     using C = RowCfg<PL>;
@@ -363,14 +364,15 @@ template <typename PL_> struct ColCfg {
 };
 
 // Passes A / C on the kx-rows [grp*SQ, grp*SQ+SQ) of one plane:
-// Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]).  The cuFFT column halves plus
+// Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]); Xout may be Xin (each wave holds
+// its whole sequence in registers before it stores).  The cuFFT column halves plus
 // modulateAndNormalize of fmvd_deconvolve_cuda.cu:327-372 (normalisation
 // folded into K on the host).  kx = 0 holds two real columns packed as
 // DC + i*Nyquist: its spectrum is split with A[k] = (C[k] + conj C[-k])/2,
 // B[k] = (C[k] - conj C[-k])/(2i), multiplied by the DC and Nyquist kernel
 // columns and recombined as A*K0 + i*B*KN.
 template <typename PL>
-__device__ __forceinline__ void col_tile(const ColGeom& g, const float2* __restrict__ Xin, float2* __restrict__ Xout,
+__device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, float2* Xout,
                                          const float2* __restrict__ K, int grp, float2* lds)
 {
     using C = ColCfg<PL>;

@@ -17,9 +17,12 @@ enum TimingKind {
 
 bool timing_on();
 
+// sample = false skips this launch: callers that issue many short launches
+// time a regular subset of them, since every recorded event pair costs
+// microseconds of GPU time between kernels.
 class TimedScope {
 public:
-    TimedScope(int kind, hipStream_t s);
+    TimedScope(int kind, hipStream_t s, bool sample = true);
     ~TimedScope() noexcept(false);
     TimedScope(const TimedScope&) = delete;
     TimedScope& operator=(const TimedScope&) = delete;

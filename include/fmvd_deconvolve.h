@@ -104,6 +104,13 @@ int fmvd_set_estimate_sink_cuda_8(struct fmvd_plan_cuda_8 *plan, fmvd_estimate_s
 int fmvd_set_batch_planes_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes);
 int fmvd_set_batch_planes_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
 
+/* Planes whose whole iteration chain runs before the next planes start
+ * (0 = automatic: as many as keep their spectra and estimates inside a
+ * 192 MiB cache budget, 8 at FFT 1536 x 1536).  Results do not depend on it.
+ * No counterpart in the reference.  Returns the value in effect. */
+int fmvd_set_chunk_planes_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes);
+int fmvd_set_chunk_planes_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
+
 /* GPU-resident volume deconvolution: d_views[v] (host array of nViews device
  * pointers) each point to nPlanes planes of dataW*dataH samples in device
  * memory.  Results go to d_out (nPlanes planes of samples) and/or d_estimate
@@ -135,7 +142,8 @@ int fmvd_autotune_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes, int *chosen,
 int fmvd_set_variants_cuda_16(struct fmvd_plan_cuda_16 *plan, const int *variants);
 int fmvd_set_variants_cuda_8(struct fmvd_plan_cuda_8 *plan, const int *variants);
 
-/* Plan geometry: fills dims[0..5] = dataH, dataW, fftH, fftW, nViews, batch planes. */
+/* Plan geometry: fills dims[0..6] = dataH, dataW, fftH, fftW, nViews, batch
+ * planes, planes per cache-resident chunk. */
 int fmvd_plan_info_cuda_16(const struct fmvd_plan_cuda_16 *plan, int *dims);
 int fmvd_plan_info_cuda_8(const struct fmvd_plan_cuda_8 *plan, int *dims);
 

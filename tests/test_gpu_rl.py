@@ -14,11 +14,11 @@ from gpu_util import check_rl, empty_device, from_device, make_case, to_device
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(fm, data, kernels, weights, psf, iters, bits, batch=0, nstreams=3):
+def run_gpu(fm, data, kernels, weights, psf, iters, bits, batch=0, nstreams=3, chunk=0):
     Z, V, dh, dw = data.shape
     dt = np.uint16 if bits == 16 else np.uint8
     with fm.Deconvolver(dh, dw, list(weights), [k.copy() for k in kernels], psf_type=psf, bits=bits,
-                        nstreams=nstreams, batch_planes=batch) as dec:
+                        nstreams=nstreams, batch_planes=batch, chunk_planes=chunk) as dec:
         planes = [[data[z, v].astype(dt) for v in range(V)] for z in range(Z)]
         out, est = dec.run(planes, iters, want_estimate=True)
         info = dec.info()
@@ -99,14 +99,15 @@ def test_batches_and_device_volume_agree(fm):
     Z, V, dh, dw, k = 7, 2, 90, 110, 11
     data, kernels, weights = make_case(Z, V, dh, dw, k, k, 16, seed=3)
     ref_out, ref_est, _ = run_gpu(fm, data, kernels, weights, 0, 6, 16, batch=1)
-    for b in (2, 3, 8):
-        out, est, _ = run_gpu(fm, data, kernels, weights, 0, 6, 16, batch=b)
-        assert np.array_equal(out, ref_out) and np.array_equal(est, ref_est), b
+    for b, c in ((2, 0), (3, 0), (8, 0), (8, 3), (7, 2), (8, 1)):
+        # c: planes per cache-resident chunk inside a batch (0 = automatic)
+        out, est, _ = run_gpu(fm, data, kernels, weights, 0, 6, 16, batch=b, chunk=c)
+        assert np.array_equal(out, ref_out) and np.array_equal(est, ref_est), (b, c)
     # GPU-resident volume through the same plan type
     dviews = [to_device(np.ascontiguousarray(data[:, v])) for v in range(V)]
     dout = empty_device((Z, dh, dw), np.uint16)
     dest = empty_device((Z, dh, dw), np.float32)
-    with fm.Deconvolver(dh, dw, list(weights), [x.copy() for x in kernels], batch_planes=3) as dec:
+    with fm.Deconvolver(dh, dw, list(weights), [x.copy() for x in kernels], batch_planes=5, chunk_planes=2) as dec:
         dec.run_device([t.ptr for t in dviews], Z, 6, out_ptr=dout.ptr, est_ptr=dest.ptr,
                        stream=0)
         fm.device_synchronize()
@@ -192,7 +193,7 @@ def test_convergence_trace_matches_oracle_estimates(fm, orc):
     ref = ests[4].astype(np.float32)
     dviews = [to_device(np.ascontiguousarray(data[:, v])) for v in range(V)]
     dref = to_device(ref)
-    with fm.Deconvolver(dh, dw, list(weights), [x.copy() for x in kernels], batch_planes=2) as dec:
+    with fm.Deconvolver(dh, dw, list(weights), [x.copy() for x in kernels], batch_planes=3, chunk_planes=2) as dec:
         errs = dec.convergence([t.ptr for t in dviews], Z, 4, dref.ptr)
     want = [np.linalg.norm(e - ests[4]) / np.linalg.norm(ests[4]) for e in ests]
     print("\\n[convergence]", errs, want)
