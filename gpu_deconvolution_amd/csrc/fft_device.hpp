@@ -290,7 +290,7 @@ template <> struct Plan<64> : PlanDef<64, 4, RadixList<16, 4>> {};
 template <> struct Plan<128> : PlanDef<128, 8, RadixList<16, 8>> {};
 template <> struct Plan<256> : PlanDef<256, 16, RadixList<16, 16>> {};
 template <> struct Plan<512> : PlanDef<512, 32, RadixList<16, 16, 2>> {};
-template <> struct Plan<768> : PlanDef<768, 32, RadixList<8, 8, 12>> {};
+template <> struct Plan<768> : PlanDef<768, 64, RadixList<4, 4, 4, 12>, 8, 4> {};
 template <> struct Plan<1024> : PlanDef<1024, 64, RadixList<16, 16, 4>> {};
 template <> struct Plan<1280> : PlanDef<1280, 64, RadixList<4, 4, 4, 20>> {};
 template <> struct Plan<1536> : PlanDef<1536, 64, RadixList<8, 8, 24>> {};
@@ -318,8 +318,8 @@ template <int N, int V> struct PlanV : Plan<N> {
 #define FMVD_PLAN_VARIANT(n, v, t, ...) FMVD_PLAN_VARIANT_G(n, v, t, 0, __VA_ARGS__)
 FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
 FMVD_PLAN_VARIANT(512, 1, 64, 8, 8, 8)
-FMVD_PLAN_VARIANT_GW(768, 1, 64, 8, 6, 4, 4, 4, 12)
-FMVD_PLAN_VARIANT_G(768, 2, 64, 8, 4, 4, 4, 12)
+FMVD_PLAN_VARIANT_GW(768, 1, 64, 4, 4, 4, 4, 4, 12)
+FMVD_PLAN_VARIANT(768, 2, 32, 8, 8, 12)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
 FMVD_PLAN_VARIANT(1280, 1, 32, 8, 8, 20)

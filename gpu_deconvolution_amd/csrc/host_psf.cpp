@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "hip_check.hpp"
+#include "rl_launch.hpp"
 
 namespace fmvd {
 
@@ -252,8 +253,9 @@ std::vector<float2> kernel_spectrum_transposed(const double* k, int kw, int kh, 
             col[2 * yi + 1] += F[(size_t)i * (N + 1) + kx].imag();
         }
         run_fft(colplan, col);
+        // device layout (rl_launch.hpp): kx pairs interleaved, row N contiguous
         for (int ky = 0; ky < H; ++ky)
-            out[(size_t)kx * H + ky] =
+            out[kx < N ? (size_t)xpos(kx, ky, H) : (size_t)N * H + ky] =
                 make_float2((float)(col[2 * ky] * scale), (float)(col[2 * ky + 1] * scale));
     });
     return out;

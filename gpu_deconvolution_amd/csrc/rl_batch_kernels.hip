@@ -105,7 +105,7 @@ __global__ void batch_const_init_kernel(const BatchSeg* __restrict__ segs, const
     if (i >= per_plane) return;
     const int e = find_entry(list, n, p);
     const float value = segs[list[e]].e0;
-    XE[p * xstride + i] = (i < H) ? make_float2(value, 0.0f) : make_float2(0.0f, 0.0f);
+    XE[p * xstride + i] = (i < 2 * H && !(i & 1)) ? make_float2(value, 0.0f) : make_float2(0.0f, 0.0f);
     reinterpret_cast<float2*>(ES + p * estride)[i] = make_float2(value, value);
 }
 

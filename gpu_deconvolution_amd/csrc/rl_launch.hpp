@@ -1,16 +1,19 @@
 // Launch interface between the RL host engine and the RL device kernels.
 //
-// Device layout of one plane's "row spectrum" (transposed, packed):
-//   X[kx][y], kx in [0, N) with N = fftW/2, y in [0, fftH), complex64,
-//   X[kx][y] = sum_x f[y][x] exp(-2 pi i kx x / fftW)   for kx in [1, N),
-//   X[0][y]  = (DC(y), Nyquist(y))  packed as (re, im),
+// Device layout of one plane's "row spectrum" (transposed, packed, with kx
+// pairs interleaved):
+//   X(kx, y) at xpos(kx, y) = (kx >> 1) * 2 fftH + 2 y + (kx & 1),
+//   kx in [0, N) with N = fftW/2, y in [0, fftH), complex64,
+//   X(kx, y) = sum_x f[y][x] exp(-2 pi i kx x / fftW)   for kx in [1, N),
+//   X(0, y)  = (DC(y), Nyquist(y))  packed as (re, im),
 // i.e. one plane occupies N*fftH complex = fftW*fftH floats.  The column pass
-// transforms each kx-row (contiguous, length fftH); the row passes read and
-// write it transposed in tiles of G rows.
+// transforms each kx sequence (stride 2 complex along y, which loads at full
+// bandwidth); the row passes read and write it transposed in tiles of G rows,
+// whose runs are 2G complex long (128 bytes at G = 8, whole cache lines).
 //
-// Kernel spectra use the same transposed layout with N+1 rows: rows 1..N-1
-// hold K[ky][kx]; row 0 holds K[ky][0] and row N holds K[ky][N] (both
-// complex), already multiplied by the pass's normalisation.
+// Kernel spectra use the same layout for kx in [0, N) (kx = 0 holding
+// K[ky][0]) followed by the contiguous row kx = N (K[ky][N]) at N*fftH, all
+// complex and already multiplied by the pass's normalisation.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -18,6 +21,11 @@
 #include <vector>
 
 namespace fmvd {
+
+__host__ __device__ __forceinline__ long xpos(int kx, int y, int H)
+{
+    return (long)(kx >> 1) * 2 * H + 2 * y + (kx & 1);
+}
 
 struct RowGeom {
     int H, W, N;          // fftH, fftW, fftW/2

@@ -105,7 +105,8 @@ __global__ void const_init_kernel(float2* __restrict__ XE, float* __restrict__ E
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
     if (i >= per_plane) return;
-    XE[p * xstride + i] = (i < H) ? make_float2(value, 0.0f) : make_float2(0.0f, 0.0f);
+    // kx = 0 (DC + i Nyquist) sits at the even positions below 2H
+    XE[p * xstride + i] = (i < 2 * H && !(i & 1)) ? make_float2(value, 0.0f) : make_float2(0.0f, 0.0f);
     reinterpret_cast<float2*>(ES + p * estride)[i] = make_float2(value, value);
 }
 

@@ -21,14 +21,15 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblocks)
     return x * q + (x < rem ? x : rem) + k;
 }
 
-// LDS row pitch >= len with 2*L = 64/G (mod 64) (4-byte banks): the
-// transposed tile access (G consecutive rows per 8-byte column element) then
-// touches distinct bank pairs across a half-wavefront.
+// LDS row pitch >= len with L = 16/G (mod 16).  The LDS serves 8-byte
+// accesses a quarter-wavefront (16 lanes, 32 four-byte banks) per cycle; in
+// the transposed tile access those lanes hold G consecutive rows times 16/G
+// consecutive columns, which then land on 16 distinct bank pairs.
 constexpr int row_pitch(int len, int G)
 {
-    const int want = (32 / G) % 32;
+    const int want = G >= 16 ? 1 : 16 / G;
     int L = len;
-    while (L % 32 != want) ++L;
+    while (L % 16 != want) ++L;
     return L;
 }
 
@@ -63,25 +64,54 @@ template <typename C, int R> constexpr bool affine_layout()
     return C::T % 16 == 0 && (C::N / R) % 16 == 0 && C::N % 16 == 0;
 }
 
-// Transposed tile load: lds[g][kx] <- X[kx][y0+g].  Thread tid owns
-// g = tid % G and kx = tid / G + k*T (k < P).
-template <typename C>
-__device__ __forceinline__ void load_tile(float2* lds, const float2* __restrict__ X, int H, int y0, int tid)
+// Tile element k of thread tid (q = tid + k * THREADS) in the spectrum
+// layout of xpos: run j = q / (2G), row g = (q % (2G)) / 2, kx = 2j + (q & 1),
+// so that 2G consecutive lanes move one 16G-byte run.  For even T a thread's
+// elements are kx = kx0 + k T at global offsets xpos(kx0, y0 + g) + k T H.
+template <typename C> __device__ __forceinline__ void tile_elem(int tid, int k, int& g, int& kx)
 {
-    const int g = tid % C::G, kx0 = tid / C::G;
-    if constexpr (C::T % 16 == 0) {
-        const unsigned off0 = (unsigned)(kx0 * H + y0 + g), step = (unsigned)(C::T * H);
-        float2 tmp[C::P];
+    const int q = tid + k * C::THREADS, e = q % (2 * C::G);
+    g = e >> 1;
+    kx = 2 * (q / (2 * C::G)) + (e & 1);
+}
+
+// Transposed tile load: lds[g][kx] <- X(kx, y0 + g).  fetch_tile issues the
+// global loads into registers and stage_tile writes them to LDS, so that a
+// caller can put further loads in flight between the two: the wait before
+// the LDS writes then covers only the tile.
+template <typename C>
+__device__ __forceinline__ void fetch_tile(float2* pre, const float2* __restrict__ X, int H, int y0, int tid)
+{
+    if constexpr (C::T % 2 == 0) {
+        int g, kx0;
+        tile_elem<C>(tid, 0, g, kx0);
+        const unsigned off0 = (unsigned)xpos(kx0, y0 + g, H), step = (unsigned)(C::T * H);
 #pragma unroll
-        for (int k = 0; k < C::P; ++k) tmp[k] = X[off0 + k * step];
+        for (int k = 0; k < C::P; ++k) pre[k] = X[off0 + k * step];
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::P; ++k) {
+            int g, kx;
+            tile_elem<C>(tid, k, g, kx);
+            pre[k] = X[(unsigned)xpos(kx, y0 + g, H)];
+        }
+    }
+}
+template <typename C>
+__device__ __forceinline__ void stage_tile(float2* lds, const float2* pre, int tid)
+{
+    if constexpr (C::T % 16 == 0) {
+        int g, kx0;
+        tile_elem<C>(tid, 0, g, kx0);
         float2* dst = lds + g * C::L + lds_pad(kx0);
 #pragma unroll
-        for (int k = 0; k < C::P; ++k) dst[k * pad_c<C::T>()] = tmp[k];
+        for (int k = 0; k < C::P; ++k) dst[k * pad_c<C::T>()] = pre[k];
     } else {
-#pragma unroll 4
+#pragma unroll
         for (int k = 0; k < C::P; ++k) {
-            const int kx = kx0 + k * C::T;
-            lds[g * C::L + lds_pad(kx)] = X[(unsigned)(kx * H + y0 + g)];
+            int g, kx;
+            tile_elem<C>(tid, k, g, kx);
+            lds[g * C::L + lds_pad(kx)] = pre[k];
         }
     }
 }
@@ -160,10 +190,11 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
         for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RLF>(t, i))] = v[i];
     }
     __syncthreads();
-    const int g = tid % C::G, kx0 = tid / C::G;
-    const float2* r = lds + g * C::L;
     if constexpr (C::T % 16 == 0) {
-        const unsigned off0 = (unsigned)(kx0 * H + y0 + g), step = (unsigned)(C::T * H);
+        int g, kx0;
+        tile_elem<C>(tid, 0, g, kx0);
+        const float2* r = lds + g * C::L;
+        const unsigned off0 = (unsigned)xpos(kx0, y0 + g, H), step = (unsigned)(C::T * H);
         const int pk = lds_pad(kx0), mk = lds_mirror(kx0);
 #pragma unroll
         for (int k = 0; k < C::P; ++k) {
@@ -182,60 +213,68 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
     } else {
 #pragma unroll 4
         for (int k = 0; k < C::P; ++k) {
-            const int kx = kx0 + k * C::T;
+            int g, kx;
+            tile_elem<C>(tid, k, g, kx);
+            const float2* r = lds + g * C::L;
             const float2 zk = r[lds_pad(kx)];
             const float2 xo = kx == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y)
                                       : r2c_combine(zk, r[lds_pad(C::N - kx)], twW[kx]);
-            X[(unsigned)(kx * H + y0 + g)] = cscale(xo, scale);
+            X[(unsigned)xpos(kx, y0 + g, H)] = cscale(xo, scale);
         }
     }
 }
 
-// Reads the two values at x = 2n and 2n+1 (n = t + c) of one padded row
-// through the pad map (fmvd_deconvolve_cuda.impl.cu:34-50, pad_index).  c is
-// the register slot's compile-time offset, so every thread of a sequence
-// takes the same branch: all positions inside the data (one paired load when
-// `pairs`), all in the region that maps to x = 0, or the per-lane map.
-template <typename T, typename PairT>
-__device__ __forceinline__ void load_padded_pair(const T* __restrict__ row, int n, int c, int span, int dataW,
-                                                 int half, bool pairs, float& a, float& b)
-{
-    if (pairs && 2 * (c + span) <= dataW) {
-        const PairT w = reinterpret_cast<const PairT*>(row)[n];
-        if constexpr (sizeof(T) == 4) {
-            a = w.x;
-            b = w.y;
-        } else {
-            constexpr unsigned bits = 8 * sizeof(T), mask = (1u << bits) - 1u;
-            a = (float)(w & mask);
-            b = (float)((unsigned)w >> bits);
-        }
-    } else if (2 * c >= dataW + half) {
-        a = b = (float)row[0];
-    } else {
-        a = (float)row[pad_index(2 * n, dataW, half)];
-        b = (float)row[pad_index(2 * n + 1, dataW, half)];
-    }
-}
+// The two values at x = 2n and 2n+1 (n = t + c) of one padded row, read
+// through the pad map (fmvd_deconvolve_cuda.impl.cu:34-50, pad_index) and
+// returned still packed (two samples in one word, or a float pair) so that
+// the load can be issued long before the values are used.  c is the register
+// slot's compile-time offset, so every thread of a sequence takes the same
+// branch: all positions inside the data (one paired load when `pairs`), all
+// in the region that maps to x = 0, or the per-lane map.
 template <typename T> struct PairOf;
 template <> struct PairOf<uint16_t> { using type = uint32_t; };
 template <> struct PairOf<uint8_t> { using type = uint16_t; };
 template <> struct PairOf<float> { using type = float2; };
 
-// Load a G-row tile and run the inverse real FFT of this thread's row:
-// on return v holds (f[2n], f[2n+1]) at the positions slot_pos<PL, RLI>.
-template <typename C>
-__device__ __forceinline__ void tile_c2r(float2* v, float2* lds, const float2* __restrict__ X, int H, int y0,
-                                         int tid, int seq, int t, const RowGeom& g)
+template <typename T, typename PairT = typename PairOf<T>::type>
+__device__ __forceinline__ PairT pack_pair(T a, T b)
 {
-    float2* row = lds + seq * C::L;
-    load_tile<C>(lds, X, H, y0, tid);
-    __syncthreads();
+    if constexpr (sizeof(T) == 4) return make_float2(a, b);
+    else return (PairT)((unsigned)a | ((unsigned)b << (8 * sizeof(T))));
+}
+
+template <typename T, typename PairT = typename PairOf<T>::type>
+__device__ __forceinline__ PairT load_padded_pair(const T* __restrict__ row, int n, int c, int span, int dataW,
+                                                  int half, bool pairs)
+{
+    if (pairs && 2 * (c + span) <= dataW) return reinterpret_cast<const PairT*>(row)[n];
+    if (2 * c >= dataW + half) return pack_pair<T>(row[0], row[0]);
+    return pack_pair<T>(row[pad_index(2 * n, dataW, half)], row[pad_index(2 * n + 1, dataW, half)]);
+}
+
+template <typename T, typename PairT>
+__device__ __forceinline__ void unpack_pair(PairT w, float& a, float& b)
+{
+    if constexpr (sizeof(T) == 4) {
+        a = w.x;
+        b = w.y;
+    } else {
+        constexpr unsigned bits = 8 * sizeof(T), mask = (1u << bits) - 1u;
+        a = (float)(w & mask);
+        b = (float)((unsigned)w >> bits);
+    }
+}
+
+// Inverse real FFT of this thread's row from the staged tile (after the
+// workgroup barrier): on return v holds (f[2n], f[2n+1]) at the positions
+// slot_pos<PL, RLI>.
+template <typename C>
+__device__ __forceinline__ void row_c2r(float2* v, float2* row, int t, const RowGeom& g)
+{
     c2r_pre<C>(v, row, t, g.twW);
     seq_sync<typename C::PL>();
     fft_regs<typename C::PL, +1>(v, row, t, g.twP[C::PL::variant]);
 }
-
 
 // ------------------------------------------------------------- row tiles
 // Each works on the G rows [y0, y0+G) of one padded plane; lds holds
@@ -250,21 +289,29 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
                                            const SAMPLE* __restrict__ dplane, int y0, float2* lds)
 {
     using C = RowCfg<PL>;
+    using PairT = typename PairOf<SAMPLE>::type;
     const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
     const int y = y0 + seq;
     float2* row = lds + seq * C::L;
     float2 v[C::P];
-    tile_c2r<C>(v, lds, Xp, g.H, y0, tid, seq, t, g);
-
+    fetch_tile<C>(v, Xp, g.H, y0, tid);
+    // the row's samples are loaded behind the tile, ahead of the FFT
     const SAMPLE* drow = dplane + (unsigned)(pad_index(y, g.dataH, g.dkY) * g.dataW);
     const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)dplane % (2 * sizeof(SAMPLE)) == 0);
+    PairT d[C::P];
 #pragma unroll
     for (int i = 0; i < C::P; ++i) {
         const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
-        const int n = t + c;
+        d[i] = load_padded_pair<SAMPLE>(drow, t + c, c, C::T, g.dataW, g.dkX, pairs);
+    }
+    stage_tile<C>(lds, v, tid);
+    __syncthreads();
+    row_c2r<C>(v, row, t, g);
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
         const float q0 = v[i].x, q1 = v[i].y;
         float d0, d1;
-        load_padded_pair<SAMPLE, typename PairOf<SAMPLE>::type>(drow, n, c, C::T, g.dataW, g.dkX, pairs, d0, d1);
+        unpack_pair<SAMPLE>(d[i], d0, d1);
         // hardware reciprocal (1 ulp) instead of the IEEE division sequence
         v[i].x = q0 > 1.0f ? d0 * __builtin_amdgcn_rcpf(q0) : q0;
         v[i].y = q1 > 1.0f ? d1 * __builtin_amdgcn_rcpf(q1) : q1;
@@ -287,26 +334,31 @@ __device__ __forceinline__ void update_tile(const RowGeom& g, float2* XEp,
     const int y = y0 + seq;
     float2* row = lds + seq * C::L;
     float2 v[C::P];
-    tile_c2r<C>(v, lds, XCp, g.H, y0, tid, seq, t, g);
-
+    fetch_tile<C>(v, XCp, g.H, y0, tid);
+    // the row's estimate is loaded behind the tile, ahead of the FFT
     float2* erow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
+    float2 e[C::P];
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) e[i] = erow[t + (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI)];
+    stage_tile<C>(lds, v, tid);
+    __syncthreads();
+    row_c2r<C>(v, row, t, g);
+
     const float* wrow = weights + (unsigned)(pad_index(y, g.dataH, g.wkY) * g.dataW);
     const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)weights % 8 == 0);
 #pragma unroll
     for (int i = 0; i < C::P; ++i) {
         const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
-        const int n = t + c;
-        const float2 e = erow[n];
         float w0, w1;
-        load_padded_pair<float, float2>(wrow, n, c, C::T, g.dataW, g.wkX, pairs, w0, w1);
+        unpack_pair<float>(load_padded_pair<float>(wrow, t + c, c, C::T, g.dataW, g.wkX, pairs), w0, w1);
         // fmvd_deconvolve_cuda.cu:387-391 operation order
-        float t0 = e.x * v[i].x, t1 = e.y * v[i].y;
-        float c0 = t0 - e.x, c1 = t1 - e.y;
+        float t0 = e[i].x * v[i].x, t1 = e[i].y * v[i].y;
+        float c0 = t0 - e[i].x, c1 = t1 - e[i].y;
         c0 *= w0;
         c1 *= w1;
-        v[i].x = e.x + c0;
-        v[i].y = e.y + c1;
-        erow[n] = v[i];
+        v[i].x = e[i].x + c0;
+        v[i].y = e[i].y + c1;
+        erow[t + c] = v[i];
     }
     fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
     r2c_post_store<C>(v, lds, XEp, g.H, y0, tid, seq, t, g.twW, 1.0f / (float)g.W);
@@ -378,13 +430,15 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
     using C = ColCfg<PL>;
     constexpr int H = C::H;
     const int seq = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    const int kx = grp * C::SQ + seq;
-    const float2* xin = Xin + (unsigned)(kx * H);
+    // wave-uniform (a sequence never spans two wavefronts here): keep it scalar
+    // so the loads below address from one SGPR base with 32-bit lane offsets
+    const int kx = C::T % 64 == 0 ? __builtin_amdgcn_readfirstlane(grp * C::SQ + seq) : grp * C::SQ + seq;
+    const float2* xin = Xin + (unsigned)xpos(kx, 0, H);  // stride 2 along y
     float2* row = lds + seq * C::L;
 
     float2 v[C::P];
 #pragma unroll
-    for (int i = 0; i < C::P; ++i) v[i] = xin[slot_pos<PL, C::RF>(t, i)];
+    for (int i = 0; i < C::P; ++i) v[i] = xin[2 * slot_pos<PL, C::RF>(t, i)];
     fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
 
     if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
@@ -394,8 +448,8 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
         }
         seq_sync<PL>();
         if (seq == 0) {
-            const float2* K0 = K;
-            const float2* KN = K + (unsigned)(g.N * H);
+            const float2* K0 = K;                         // kx = 0, stride 2
+            const float2* KN = K + (unsigned)(g.N * H);  // kx = N, contiguous
 #pragma unroll
             for (int i = 0; i < C::P; ++i) {
                 const int k = slot_pos<PL, C::RL>(t, i);
@@ -404,26 +458,26 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
                 const float2 a = cscale(cadd(ck, cm), 0.5f);
                 const float2 d = csub(ck, cm);
                 const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
-                const float2 ak = cmul(a, K0[k]);
+                const float2 ak = cmul(a, K0[2 * k]);
                 const float2 bk = cmul(b, KN[k]);
                 v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
             }
         } else {
-            const float2* Kr = K + (unsigned)(kx * H);
+            const float2* Kr = K + (unsigned)xpos(kx, 0, H);
 #pragma unroll
-            for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[slot_pos<PL, C::RL>(t, i)]);
+            for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[2 * slot_pos<PL, C::RL>(t, i)]);
         }
         seq_sync<PL>();
     } else {
-        const float2* Kr = K + (unsigned)(kx * H);
+        const float2* Kr = K + (unsigned)xpos(kx, 0, H);
 #pragma unroll
-        for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[slot_pos<PL, C::RL>(t, i)]);
+        for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[2 * slot_pos<PL, C::RL>(t, i)]);
     }
 
     fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);
-    float2* xo = Xout + (unsigned)(kx * H);
+    float2* xo = Xout + (unsigned)xpos(kx, 0, H);
 #pragma unroll
-    for (int i = 0; i < C::P; ++i) xo[slot_pos<PL, C::RF>(t, i)] = v[i];
+    for (int i = 0; i < C::P; ++i) xo[2 * slot_pos<PL, C::RF>(t, i)] = v[i];
 }
 
 }  // namespace fmvd
