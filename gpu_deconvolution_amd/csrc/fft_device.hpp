@@ -318,14 +318,14 @@ template <int N, int V> struct PlanV : Plan<N> {
 #define FMVD_PLAN_VARIANT(n, v, t, ...) FMVD_PLAN_VARIANT_G(n, v, t, 0, __VA_ARGS__)
 FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
 FMVD_PLAN_VARIANT(512, 1, 64, 8, 8, 8)
-FMVD_PLAN_VARIANT_GW(768, 1, 64, 4, 4, 4, 4, 4, 12)
-FMVD_PLAN_VARIANT(768, 2, 32, 8, 8, 12)
+FMVD_PLAN_VARIANT_GW(768, 1, 64, 4, 5, 4, 4, 4, 12)
+FMVD_PLAN_VARIANT_GW(768, 2, 64, 4, 6, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
 FMVD_PLAN_VARIANT(1280, 1, 32, 8, 8, 20)
 FMVD_PLAN_VARIANT(1280, 2, 64, 4, 4, 4, 4, 5)
 FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
-FMVD_PLAN_VARIANT(1536, 2, 64, 4, 4, 8, 12)
+FMVD_PLAN_VARIANT(1536, 2, 128, 4, 4, 4, 4, 6)
 FMVD_PLAN_VARIANT(2560, 1, 64, 4, 8, 8, 10)
 #undef FMVD_PLAN_VARIANT
 #undef FMVD_PLAN_VARIANT_G

@@ -295,7 +295,8 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
     float2* row = lds + seq * C::L;
     float2 v[C::P];
     fetch_tile<C>(v, Xp, g.H, y0, tid);
-    // the row's samples are loaded behind the tile, ahead of the FFT
+    // the row's samples (streamed from HBM once per view-iteration) are loaded
+    // behind the tile, ahead of the FFT, so their latency hides behind it
     const SAMPLE* drow = dplane + (unsigned)(pad_index(y, g.dataH, g.dkY) * g.dataW);
     const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)dplane % (2 * sizeof(SAMPLE)) == 0);
     PairT d[C::P];
@@ -335,29 +336,26 @@ __device__ __forceinline__ void update_tile(const RowGeom& g, float2* XEp,
     float2* row = lds + seq * C::L;
     float2 v[C::P];
     fetch_tile<C>(v, XCp, g.H, y0, tid);
-    // the row's estimate is loaded behind the tile, ahead of the FFT
-    float2* erow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
-    float2 e[C::P];
-#pragma unroll
-    for (int i = 0; i < C::P; ++i) e[i] = erow[t + (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI)];
     stage_tile<C>(lds, v, tid);
     __syncthreads();
     row_c2r<C>(v, row, t, g);
 
+    float2* erow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
     const float* wrow = weights + (unsigned)(pad_index(y, g.dataH, g.wkY) * g.dataW);
     const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)weights % 8 == 0);
 #pragma unroll
     for (int i = 0; i < C::P; ++i) {
         const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
+        const float2 e = erow[t + c];
         float w0, w1;
         unpack_pair<float>(load_padded_pair<float>(wrow, t + c, c, C::T, g.dataW, g.wkX, pairs), w0, w1);
         // fmvd_deconvolve_cuda.cu:387-391 operation order
-        float t0 = e[i].x * v[i].x, t1 = e[i].y * v[i].y;
-        float c0 = t0 - e[i].x, c1 = t1 - e[i].y;
+        float t0 = e.x * v[i].x, t1 = e.y * v[i].y;
+        float c0 = t0 - e.x, c1 = t1 - e.y;
         c0 *= w0;
         c1 *= w1;
-        v[i].x = e[i].x + c0;
-        v[i].y = e[i].y + c1;
+        v[i].x = e.x + c0;
+        v[i].y = e.y + c1;
         erow[t + c] = v[i];
     }
     fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
