@@ -51,11 +51,11 @@ def parse():
     p.add_argument("--train-samples", type=int, default=2000)
     p.add_argument("--epochs", type=int, default=3)
     p.add_argument("--traffic-run", action="store_true",
-                   help="short run for HBM counter collection: 64 planes, 1 iteration, 1 step, no CPU "
+                   help="short run for HBM counter collection: 64 planes, 3 iterations, 1 step, no CPU "
                         "baseline, then a 1 GiB device-to-device copy as the counters' calibration kernel")
     args = p.parse_args()
     if args.traffic_run:
-        args.planes, args.iterations, args.steps, args.warmup, args.no_cpu, args.no_batch = 64, 1, 1, 0, True, True
+        args.planes, args.iterations, args.steps, args.warmup, args.no_cpu, args.no_batch = 64, 3, 1, 0, True, True
         args.no_learned = True
     return args
 
@@ -295,7 +295,8 @@ def main():
     tr_ms, tr_n = timing["transform"]
     tr_gbs = (2 * Z * H * W * 2) * tr_n / (tr_ms / 1000.0) / 1e9 if tr_n else None
     for k, s in kstats.items():
-        log(f"[bench] {k:7s} {s['launches']:5d} launches avg {s['avg_ms']:.3f} ms  {s['achieved_gbs']:.0f} GB/s")
+        log(f"[bench] {k:7s} {s['launches']:5d} timed launches of {s['planes_per_launch']} planes, avg "
+            f"{1000 * s['avg_ms']:.1f} us  {s['achieved_gbs']:.0f} GB/s")
     log(f"[bench] RL chain (B_RL = 64S+10P = {B_RL / 1e6:.2f} MB/view-iter-plane): {iter_gbs:.0f} GB/s; "
         f"transform {tr_gbs and round(tr_gbs)} GB/s")
 

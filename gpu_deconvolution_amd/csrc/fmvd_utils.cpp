@@ -97,20 +97,35 @@ void drain_locked()
 
 bool timing_on() { return g_timing; }
 
+thread_local hipEvent_t t_start = nullptr, t_stop = nullptr;
+
+void take_launch_events(hipEvent_t& start, hipEvent_t& stop)
+{
+    start = t_start;
+    stop = t_stop;
+    t_start = t_stop = nullptr;
+}
+
 TimedScope::TimedScope(int kind, hipStream_t s, bool sample) : kind_(kind), s_(s), on_(g_timing && sample)
 {
     if (!on_) return;
     std::lock_guard<std::mutex> lk(g_tmu);
     a_ = pool_get();
     b_ = pool_get();
-    FMVD_HIP(hipEventRecord(a_, s_));
+    t_start = a_;
+    t_stop = b_;
 }
 
 TimedScope::~TimedScope() noexcept(false)
 {
     if (!on_) return;
     std::lock_guard<std::mutex> lk(g_tmu);
-    FMVD_HIP(hipEventRecord(b_, s_));
+    if (t_start == a_) {  // the scope launched nothing: no sample
+        t_start = t_stop = nullptr;
+        g_pool.push_back(a_);
+        g_pool.push_back(b_);
+        return;
+    }
     g_pending.push_back({kind_, a_, b_});
     if (g_pending.size() > 4096) drain_locked();
 }

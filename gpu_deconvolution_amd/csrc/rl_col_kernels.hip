@@ -16,6 +16,7 @@
 #include "hip_check.hpp"
 #include "rl_launch.hpp"
 #include "rl_tiles.hpp"
+#include "timing.hpp"
 
 namespace fmvd {
 
@@ -70,7 +71,7 @@ static void launch_col_plan(const ColGeom& g, const float2* Xin, float2* Xout, l
     using C = ColCfg<PL>;
     if (g.N % C::SQ) throw_msg("row count not divisible by column group");
     dim3 grid((unsigned)(g.N / C::SQ * nplanes));
-    col_conv_kernel<PL><<<grid, C::THREADS, 0, s>>>(g, Xin, Xout, xstride, K, nplanes);
+    launch(col_conv_kernel<PL>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes);
 }
 
 template <int H>

@@ -288,7 +288,7 @@ template <typename S> struct Plan {
         auto time_it = [&](auto&& launch) -> float {
             launch();  // warm-up
             FMVD_HIP(hipEventRecord(a, s));
-            const int reps = 3;
+            const int reps = 10;
             for (int r = 0; r < reps; ++r) launch();
             FMVD_HIP(hipEventRecord(b, s));
             FMVD_HIP(hipEventSynchronize(b));

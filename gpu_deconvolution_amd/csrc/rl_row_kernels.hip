@@ -17,6 +17,7 @@
 #include "hip_check.hpp"
 #include "rl_launch.hpp"
 #include "rl_tiles.hpp"
+#include "timing.hpp"
 
 namespace fmvd {
 
@@ -225,8 +226,8 @@ void launch_row_ratio(const RowGeom& g, float2* X, long xstride, const SAMPLE* d
                       int variant, hipStream_t s)
 {
     dispatch_row(g.N, variant, [&]<typename PL>() {
-        row_ratio_kernel<SAMPLE, PL><<<row_grid<PL>(g.H, nplanes), RowCfg<PL>::THREADS, 0, s>>>(
-            g, X, xstride, data, dstride, nplanes);
+        launch(row_ratio_kernel<SAMPLE, PL>, row_grid<PL>(g.H, nplanes), dim3(RowCfg<PL>::THREADS), 0, s, g, X,
+               xstride, data, dstride, nplanes);
     });
     FMVD_LAUNCH_CHECK();
 }
@@ -235,8 +236,8 @@ void launch_row_update(const RowGeom& g, float2* XE, const float2* XC, float* ES
                        const float* weights, int nplanes, int variant, hipStream_t s)
 {
     dispatch_row(g.N, variant, [&]<typename PL>() {
-        row_update_kernel<PL><<<row_grid<PL>(g.H, nplanes), RowCfg<PL>::THREADS, 0, s>>>(
-            g, XE, XC, ES, xstride, estride, weights, nplanes);
+        launch(row_update_kernel<PL>, row_grid<PL>(g.H, nplanes), dim3(RowCfg<PL>::THREADS), 0, s, g, XE, XC, ES,
+               xstride, estride, weights, nplanes);
     });
     FMVD_LAUNCH_CHECK();
 }
@@ -247,7 +248,7 @@ void launch_final(const RowGeom& g, const float* ES, long estride, SAMPLE* out, 
 {
     if (nplanes <= 0) return;
     dim3 grid((unsigned)((g.dataW + 255) / 256), (unsigned)g.dataH, (unsigned)nplanes);
-    final_kernel<SAMPLE><<<grid, 256, 0, s>>>(ES, estride, g.W, g.dataH, g.dataW, out, est, ostride);
+    launch(final_kernel<SAMPLE>, grid, dim3(256), 0, s, ES, estride, g.W, g.dataH, g.dataW, out, est, ostride);
     FMVD_LAUNCH_CHECK();
 }
 
@@ -255,8 +256,8 @@ void launch_row_init(const RowGeom& g, float2* XE, float* ES, long xstride, long
                      long e0stride, int nplanes, int variant, hipStream_t s)
 {
     dispatch_row(g.N, variant, [&]<typename PL>() {
-        row_init_kernel<PL><<<row_grid<PL>(g.H, nplanes), RowCfg<PL>::THREADS, 0, s>>>(
-            g, XE, ES, xstride, estride, e0, e0stride, nplanes);
+        launch(row_init_kernel<PL>, row_grid<PL>(g.H, nplanes), dim3(RowCfg<PL>::THREADS), 0, s, g, XE, ES,
+               xstride, estride, e0, e0stride, nplanes);
     });
     FMVD_LAUNCH_CHECK();
 }
@@ -266,7 +267,7 @@ void launch_const_init(const RowGeom& g, float2* XE, float* ES, long xstride, lo
 {
     const long per_plane = (long)g.N * g.H;
     dim3 grid((unsigned)((per_plane + 255) / 256), (unsigned)nplanes);
-    const_init_kernel<<<grid, 256, 0, s>>>(XE, ES, xstride, estride, per_plane, g.H, value);
+    launch(const_init_kernel, grid, dim3(256), 0, s, XE, ES, xstride, estride, per_plane, g.H, value);
     FMVD_LAUNCH_CHECK();
 }
 

@@ -1,6 +1,10 @@
-// Optional per-kernel timing: HIP events recorded on the launch stream
-// around each kernel launch while timing is enabled.
+// Optional per-kernel timing.  While timing is enabled, a TimedScope hands
+// a start/stop event pair to the launch inside it (fmvd::launch), which
+// attaches them to the kernel's dispatch packet (hipExtLaunchKernelGGL): the
+// events then bracket the kernel's execution itself, without the gap a
+// separate event marker would add before a short kernel.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 namespace fmvd {
@@ -33,5 +37,19 @@ private:
     bool on_;
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
+
+// The event pair of the innermost open TimedScope on this thread, taken by
+// the first launch inside it (both null when none is pending).
+void take_launch_events(hipEvent_t& start, hipEvent_t& stop);
+
+// Kernel launch that picks up a pending TimedScope's events.
+template <typename... KArgs, typename... Args>
+inline void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, unsigned shmem, hipStream_t s, Args... args)
+{
+    hipEvent_t a, b;
+    take_launch_events(a, b);
+    if (a) hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, static_cast<KArgs>(args)...);
+    else kernel<<<grid, block, shmem, s>>>(static_cast<KArgs>(args)...);
+}
 
 }  // namespace fmvd

@@ -180,7 +180,7 @@ static void launch_transform(const S* src, int w, int h, int d, S* dst, int tw, 
     if (nz <= 0 || tw <= 0 || th <= 0) return;
     dim3 grid((unsigned)((tw + kTX - 1) / kTX), (unsigned)((th + kTY - 1) / kTY), (unsigned)((nz + kTZ - 1) / kTZ));
     TimedScope ts(kTimeTransform, s);
-    transform_kernel<S><<<grid, 256, 0, s>>>(src, w, h, d, dst, tw, th, z0, nz, M);
+    launch(transform_kernel<S>, grid, dim3(256), 0, s, src, w, h, d, dst, tw, th, z0, nz, M);
     FMVD_LAUNCH_CHECK();
 }
 
