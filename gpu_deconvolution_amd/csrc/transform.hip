@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -55,7 +56,16 @@ __device__ __forceinline__ S transform_sample(int w, int h, int d, const Mat12& 
     }
     constexpr double maxv = (double)((1u << (8 * sizeof(S))) - 1u);
     constexpr double scale = (double)(1u << (8 * sizeof(S)));
-    const double yv = (val / maxv) * scale + 0.5;
+    // yv = (val / maxv) * scale + 0.5 decides the sample only through its
+    // integer part and the bounds 0 and maxv (integers).  val * (1 / maxv)
+    // stays within 3 ulp of val / maxv, so the reciprocal form moves yv
+    // (< 2^16) by less than 1e-10: when it lies farther than 1e-9 from an
+    // integer both forms give the same sample, otherwise the division
+    // decides.
+    constexpr double inv_maxv = 1.0 / maxv;
+    double yv = (val * inv_maxv) * scale + 0.5;
+    const double fr = yv - floor(yv);
+    if (!(fr > 1e-9 && fr < 1.0 - 1e-9)) yv = (val / maxv) * scale + 0.5;
     if (yv >= maxv) return (S)maxv;
     if (yv > 0.0) return (S)floor(yv);
     return (S)0;
@@ -68,18 +78,21 @@ __device__ __forceinline__ S transform_sample(int w, int h, int d, const Mat12& 
 // holds every texel an in-range sample can read.  When the box fits the
 // LDS brick it is staged with row-contiguous (coalesced) loads, zero outside
 // the volume, and all trilinear fetches come from LDS; otherwise the
-// workgroup gathers from global memory.
+// workgroup gathers from global memory.  The brick (dynamic LDS, `cap`
+// texels) is sized on the host from the map: every tile's box fits it, and a
+// smaller brick lets more workgroups share a CU.
 constexpr int kTX = 16, kTY = 16, kTZ = 8;
-constexpr int kBrickBytes = 48 * 1024;
+constexpr int kBrickMaxBytes = 48 * 1024;
 
 template <typename S>
 __global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ src, int w, int h, int d,
                                                         S* __restrict__ dst, int tw, int th, int z0, int nz,
-                                                        Mat12 M)
+                                                        Mat12 M, int cap)
 {
 #pragma clang fp contract(off)
-    constexpr int CAP = kBrickBytes / (int)sizeof(S);
-    __shared__ S brick[CAP];
+    extern __shared__ __align__(16) unsigned char brick_lds[];
+    S* brick = reinterpret_cast<S*>(brick_lds);
+    const int CAP = cap;
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, zt0 = z0 + blockIdx.z * kTZ;
     const int x1 = min(x0 + kTX, tw) - 1, y1 = min(y0 + kTY, th) - 1, zt1 = min(zt0 + kTZ, z0 + nz) - 1;
@@ -115,10 +128,39 @@ __global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ sr
     }
     if (cells <= CAP) {
         const int BX = bn[0], BY = bn[1];
-        for (int e = threadIdx.x; e < (int)cells; e += 256) {
-            const int gx = b0[0] + e % BX, gy = b0[1] + (e / BX) % BY, gz = b0[2] + e / (BX * BY);
-            const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < w && gy < h && gz < d;
-            brick[e] = in ? src[((long)gz * h + gy) * w + gx] : (S)0;
+        // 16 texels per thread in flight before their LDS writes.  The brick
+        // index e splits into (x, y, z) with float reciprocals and one
+        // correction step (e < 2^16: the quotient is off by at most one).
+        constexpr int kU = 16;
+        const float inv_bx = 1.0f / (float)BX, inv_by = 1.0f / (float)BY;
+        for (int e0 = threadIdx.x; e0 < (int)cells; e0 += 256 * kU) {
+            S v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int e = e0 + 256 * u;
+                int q = (int)((float)e * inv_bx), ex = e - q * BX;
+                if (ex < 0) {
+                    --q;
+                    ex += BX;
+                } else if (ex >= BX) {
+                    ++q;
+                    ex -= BX;
+                }
+                int ez = (int)((float)q * inv_by), ey = q - ez * BY;
+                if (ey < 0) {
+                    --ez;
+                    ey += BY;
+                } else if (ey >= BY) {
+                    ++ez;
+                    ey -= BY;
+                }
+                const int gx = b0[0] + ex, gy = b0[1] + ey, gz = b0[2] + ez;
+                const bool in = e < (int)cells && gx >= 0 && gy >= 0 && gz >= 0 && gx < w && gy < h && gz < d;
+                v[u] = in ? src[((long)gz * h + gy) * w + gx] : (S)0;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (e0 + 256 * u < (int)cells) brick[e0 + 256 * u] = v[u];
         }
         __syncthreads();
         if (!active) return;
@@ -179,8 +221,22 @@ static void launch_transform(const S* src, int w, int h, int d, S* dst, int tw, 
 {
     if (nz <= 0 || tw <= 0 || th <= 0) return;
     dim3 grid((unsigned)((tw + kTX - 1) / kTX), (unsigned)((th + kTY - 1) / kTY), (unsigned)((nz + kTZ - 1) / kTZ));
+    // Box of a tile's source footprint per axis: the corner images span
+    // sum_j |m_aj| (tile extent_j - 1); flooring and the one-texel widening
+    // on both sides add at most 4 cells (+1 for rounding of the span).
+    const int ext[3] = {kTX, kTY, kTZ};
+    double cells = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        double span = 0.0;
+        for (int j = 0; j < 3; ++j) span += std::fabs(M.m[4 * a + j]) * (double)(ext[j] - 1);
+        cells *= std::ceil(span) + 5.0;
+    }
+    const double maxb = (double)kBrickMaxBytes;
+    const int bytes = cells * sizeof(S) >= maxb ? kBrickMaxBytes
+                                                : (int)((cells * sizeof(S) + 255.0) / 256.0) * 256;
     TimedScope ts(kTimeTransform, s);
-    launch(transform_kernel<S>, grid, dim3(256), 0, s, src, w, h, d, dst, tw, th, z0, nz, M);
+    launch(transform_kernel<S>, grid, dim3(256), (unsigned)bytes, s, src, w, h, d, dst, tw, th, z0, nz, M,
+           bytes / (int)sizeof(S));
     FMVD_LAUNCH_CHECK();
 }
 
