@@ -181,20 +181,23 @@ template <typename S> struct Plan {
 
     // Cache-resident chunks.  Every plane's RL is independent, so run_batch
     // splits its planes into chunks and runs each chunk's whole iteration
-    // chain before the next.  A chunk's X and ES (19 MB per plane at FFT
-    // 1536^2), its views' samples and one view's kernel spectra and weights
-    // then fit the 256 MB memory-side cache, so each pass finds most of its
-    // operands there instead of in HBM.  The default budget gives 8 planes
-    // per chunk at FFT 1536^2: 1536 row tiles and 1536 column groups per
-    // launch, whole multiples of the resident workgroups.
+    // chain before the next.  One iteration of a chunk touches its X and ES
+    // (19 MB per plane at FFT 1536^2) and samples of every view, plus every
+    // view's two kernel spectra and weights; the chunk is sized so that this
+    // fits a 200 MiB budget in the 256 MB memory-side cache, and rounded down
+    // to a power of two so that launches are whole multiples of the resident
+    // workgroups.  At FFT 1536^2 with 4 views: 4 planes, 1536 row tiles.
     int chunk_planes() const
     {
         if (chunkCap > 0) return chunkCap;
-        const size_t budget = env_bytes("FMVD_CHUNK_BYTES", (size_t)192 << 20);
-        const size_t shared = 2 * xplane() * sizeof(float2) + plane() * sizeof(float);  // K_v, Khat_v, w_v
-        const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float) + plane() * sizeof(S);
+        const size_t budget = env_bytes("FMVD_CHUNK_BYTES", (size_t)200 << 20);
+        const size_t shared = (size_t)nViews * (2 * xplane() * sizeof(float2) + plane() * sizeof(float));
+        const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float) + (size_t)nViews * plane() * sizeof(S);
         const size_t room = budget > shared ? budget - shared : per;
-        return (int)std::max<size_t>(1, std::min<size_t>(room / per, 4096));
+        size_t n = std::max<size_t>(1, std::min<size_t>(room / per, 4096));
+        size_t p2 = 1;
+        while (p2 * 2 <= n) p2 *= 2;
+        return (int)p2;
     }
 
     void run_batch(const S* const* views, long dstride, int np, int iterations, const float* init,
