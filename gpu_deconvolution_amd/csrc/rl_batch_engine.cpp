@@ -8,16 +8,21 @@
 // so a batched volume's result is bit-identical to running it alone with the
 // same FFT plan variants.
 //
-// Volumes are grouped by padded FFT extent into classes.  Per iteration the
-// device runs, for each class and view index v, the four passes (column,
-// ratio, column, update) over every plane of every class volume with more
-// than v views: one launch per pass, class and view, all captured together
-// into one graph that is replayed once per iteration.
+// Volumes are grouped by padded FFT extent into classes.  A class's planes
+// are split into cache-resident chunks (as the single-volume plan does): a
+// chunk holds whole volumes or plane ranges of one (segments of the device
+// table), sized so that one iteration's working set fits a 200 MiB budget in
+// the memory-side cache.  Per iteration of a chunk the device runs, for each
+// view index v, the four passes (column, ratio, column, update) over every
+// plane of every chunk segment with more than v views: one launch per pass
+// and view, captured into one graph per chunk that is replayed once per
+// iteration before the next chunk starts.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <vector>
@@ -47,30 +52,74 @@ template <typename S> struct Batch {
     struct Cls {
         int fftH = 0, fftW = 0, N = 0, nplanes = 0, maxV = 0, maxH = 0, maxW = 0;
         std::vector<int> vols;
+        std::vector<int> segs;   // device segment table entries of the class
+        std::vector<int> chunks;
         DeviceBuffer XE, ES, twW, twRow[3], twCol[3];  // XE: the in-place spectrum of every pass
         Variants var;
         Launch planes;
-        std::vector<Launch> col, ratio, update;  // per view index
         long xs() const { return (long)N * fftH; }
         long es() const { return (long)fftW * fftH; }
+    };
+    struct Chunk {
+        int cls = 0;
+        std::vector<int> segs;
+        std::vector<Launch> col, ratio, update;  // per view index
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
     };
 
     int device = 0;
     std::vector<Vol> vols;
     std::vector<Cls> cls;
+    std::vector<BatchSeg> table;  // host copy of the device segment table
+    std::vector<Chunk> chunks;
     DeviceBuffer segs, lists;
     std::unique_ptr<Stream> cap;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
 
     ~Batch() { drop_graph(); }
 
     void drop_graph()
     {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        for (Chunk& ch : chunks) {
+            if (ch.exec) (void)hipGraphExecDestroy(ch.exec);
+            if (ch.graph) (void)hipGraphDestroy(ch.graph);
+            ch.exec = nullptr;
+            ch.graph = nullptr;
+        }
+    }
+
+    static size_t chunk_budget()
+    {
+        const char* e = getenv("FMVD_CHUNK_BYTES");
+        return (e && *e) ? (size_t)strtoull(e, nullptr, 10) : ((size_t)200 << 20);
+    }
+
+    // The segment-table entry of a whole volume (init narrows it to planes).
+    BatchSeg volume_seg(const Vol& v) const
+    {
+        BatchSeg s{};
+        s.dataH = v.dataH;
+        s.dataW = v.dataW;
+        // data pad half-widths swapped, as the reference
+        // (fmvd_deconvolve.impl.cpp:36-37, fmvd_deconvolve_cuda.impl.cu:76-77)
+        s.dkY = v.kW / 2;
+        s.dkX = v.kH / 2;
+        s.wkY = v.kH / 2;
+        s.wkX = v.kW / 2;
+        s.nplanes = v.nplanes;
+        s.plane0 = v.plane0;
+        s.nViews = v.nViews;
+        s.e0 = 10.0f * (float)v.nViews;
+        s.dstride = (long)v.dataH * v.dataW;
+        for (int k = 0; k < v.nViews; ++k) {
+            s.data[k] = v.views[k];
+            s.w[k] = v.Wn[k].template as<float>();
+            s.K[k] = v.K[k].template as<float2>();
+            s.Kh[k] = v.Kh[k].template as<float2>();
+        }
+        s.out = v.out;
+        s.est = v.est;
+        return s;
     }
 
     void init(int n, const fmvd_batch_volume* d)
@@ -159,33 +208,47 @@ template <typename S> struct Batch {
             c.XE.alloc(sizeof(float2) * (size_t)c.xs() * c.nplanes);
             c.ES.alloc(sizeof(float) * (size_t)c.es() * c.nplanes);
         }
-        // device segment table
-        std::vector<BatchSeg> table(n);
-        for (int i = 0; i < n; ++i) {
-            const Vol& v = vols[i];
-            BatchSeg& s = table[i];
-            s = BatchSeg{};
-            s.dataH = v.dataH;
-            s.dataW = v.dataW;
-            // data pad half-widths swapped, as the reference
-            // (fmvd_deconvolve.impl.cpp:36-37, fmvd_deconvolve_cuda.impl.cu:76-77)
-            s.dkY = v.kW / 2;
-            s.dkX = v.kH / 2;
-            s.wkY = v.kH / 2;
-            s.wkX = v.kW / 2;
-            s.nplanes = v.nplanes;
-            s.plane0 = v.plane0;
-            s.nViews = v.nViews;
-            s.e0 = 10.0f * (float)v.nViews;
-            s.dstride = (long)v.dataH * v.dataW;
-            for (int k = 0; k < v.nViews; ++k) {
-                s.data[k] = v.views[k];
-                s.w[k] = v.Wn[k].template as<float>();
-                s.K[k] = v.K[k].template as<float2>();
-                s.Kh[k] = v.Kh[k].template as<float2>();
+        // device segment table: every volume split at chunk boundaries
+        const size_t budget = chunk_budget();
+        for (size_t ci = 0; ci < cls.size(); ++ci) {
+            Cls& c = cls[ci];
+            size_t used = 0;
+            auto open_chunk = [&] {
+                Chunk ch;
+                ch.cls = (int)ci;
+                c.chunks.push_back((int)chunks.size());
+                chunks.push_back(std::move(ch));
+                used = 0;
+            };
+            open_chunk();
+            for (int i : c.vols) {
+                const Vol& v = vols[i];
+                const size_t plane = (size_t)v.dataH * v.dataW;
+                // per plane: spectrum, estimate, every view's samples; per
+                // volume: every view's two kernel spectra and weights
+                const size_t pp = (size_t)c.xs() * sizeof(float2) + (size_t)c.es() * sizeof(float) +
+                                  (size_t)v.nViews * plane * sizeof(S);
+                const size_t pv = (size_t)v.nViews * (2 * (size_t)c.xs() * sizeof(float2) + plane * sizeof(float));
+                int first = 0;
+                while (first < v.nplanes) {
+                    const size_t need = pv + pp;
+                    if (used + need > budget && !chunks[c.chunks.back()].segs.empty()) open_chunk();
+                    const size_t room = budget > used + pv ? budget - used - pv : 0;
+                    const int take = std::max(1, std::min(v.nplanes - first, (int)std::min<size_t>(room / pp, 1 << 20)));
+                    BatchSeg sg = volume_seg(v);
+                    sg.nplanes = take;
+                    sg.plane0 = v.plane0 + first;
+                    for (int k = 0; k < v.nViews; ++k)
+                        sg.data[k] = static_cast<const S*>(v.views[k]) + (size_t)first * plane;
+                    sg.out = v.out ? static_cast<void*>(v.out + (size_t)first * plane) : nullptr;
+                    sg.est = v.est ? v.est + (size_t)first * plane : nullptr;
+                    c.segs.push_back((int)table.size());
+                    chunks[c.chunks.back()].segs.push_back((int)table.size());
+                    table.push_back(sg);
+                    used += pv + (size_t)take * pp;
+                    first += take;
+                }
             }
-            s.out = v.out;
-            s.est = v.est;
         }
         upload(segs, table);
         build_lists();
@@ -196,16 +259,16 @@ template <typename S> struct Batch {
     void build_lists()
     {
         std::vector<int> L;
-        auto add = [&](const Cls& c, int minViews, auto blocksOf) {
+        auto add = [&](const std::vector<int>& ids_in, int minViews, auto blocksOf) {
             Launch l;
             l.off = (int)L.size();
             std::vector<int> ids, starts;
             int total = 0;
-            for (int i : c.vols)
-                if (vols[i].nViews > minViews) {
+            for (int i : ids_in)
+                if (table[i].nViews > minViews) {
                     ids.push_back(i);
                     starts.push_back(total);
-                    total += blocksOf(vols[i]);
+                    total += blocksOf(table[i]);
                 }
             starts.push_back(total);
             l.n = (int)ids.size();
@@ -215,18 +278,21 @@ template <typename S> struct Batch {
             return l;
         };
         for (Cls& c : cls) {
-            c.planes = add(c, 0, [](const Vol& v) { return v.nplanes; });
+            c.planes = add(c.segs, 0, [](const BatchSeg& g) { return g.nplanes; });
             const int Gr = batch_row_rows_per_block(c.N, c.var.ratio);
             const int Gu = batch_row_rows_per_block(c.N, c.var.update);
             const int SQ = batch_col_seqs_per_block(c.fftH, c.var.column);
             if (c.fftH % Gr || c.fftH % Gu || c.N % SQ) throw_msg("batched RL: tile shape does not divide the extent");
-            c.col.clear();
-            c.ratio.clear();
-            c.update.clear();
-            for (int k = 0; k < c.maxV; ++k) {
-                c.col.push_back(add(c, k, [&](const Vol& v) { return v.nplanes * (c.N / SQ); }));
-                c.ratio.push_back(add(c, k, [&](const Vol& v) { return v.nplanes * (c.fftH / Gr); }));
-                c.update.push_back(add(c, k, [&](const Vol& v) { return v.nplanes * (c.fftH / Gu); }));
+            for (int chi : c.chunks) {
+                Chunk& ch = chunks[chi];
+                ch.col.clear();
+                ch.ratio.clear();
+                ch.update.clear();
+                for (int k = 0; k < c.maxV; ++k) {
+                    ch.col.push_back(add(ch.segs, k, [&](const BatchSeg& g) { return g.nplanes * (c.N / SQ); }));
+                    ch.ratio.push_back(add(ch.segs, k, [&](const BatchSeg& g) { return g.nplanes * (c.fftH / Gr); }));
+                    ch.update.push_back(add(ch.segs, k, [&](const BatchSeg& g) { return g.nplanes * (c.fftH / Gu); }));
+                }
             }
         }
         // the previous lists may still be read by queued launches
@@ -264,25 +330,25 @@ template <typename S> struct Batch {
             launch_batch_const_init(sg, L(c.planes), c.planes.n, c.nplanes, c.XE.template as<float2>(),
                                     c.ES.template as<float>(), c.xs(), c.es(), c.N, c.fftH, s);
     }
-    void enqueue_iteration(hipStream_t s)
+    void enqueue_iteration(const Chunk& ch, hipStream_t s)
     {
         const BatchSeg* sg = segs.as<BatchSeg>();
-        for (Cls& c : cls) {
-            const RowGeom rg = row_geom(c);
-            const ColGeom cg = col_geom(c);
-            float2* xe = c.XE.template as<float2>();
-            float2* xt = xe;
-            float* es = c.ES.template as<float>();
-            for (int v = 0; v < c.maxV; ++v) {
-                launch_batch_col(cg, sg, L(c.col[v]), c.col[v].n, c.col[v].blocks, v, 0, xe, xt, c.xs(), c.var.column,
-                                 s);
-                launch_batch_ratio<S>(rg, sg, L(c.ratio[v]), c.ratio[v].n, c.ratio[v].blocks, v, xt, c.xs(),
-                                      c.var.ratio, s);
-                launch_batch_col(cg, sg, L(c.col[v]), c.col[v].n, c.col[v].blocks, v, 1, xt, xt, c.xs(), c.var.column,
-                                 s);
-                launch_batch_update(rg, sg, L(c.update[v]), c.update[v].n, c.update[v].blocks, v, xe, xt, es, c.xs(),
-                                    c.es(), c.var.update, s);
-            }
+        Cls& c = cls[ch.cls];
+        const RowGeom rg = row_geom(c);
+        const ColGeom cg = col_geom(c);
+        float2* xe = c.XE.template as<float2>();
+        float2* xt = xe;
+        float* es = c.ES.template as<float>();
+        for (int v = 0; v < c.maxV; ++v) {
+            if (ch.col[v].n == 0) continue;
+            launch_batch_col(cg, sg, L(ch.col[v]), ch.col[v].n, ch.col[v].blocks, v, 0, xe, xt, c.xs(), c.var.column,
+                             s);
+            launch_batch_ratio<S>(rg, sg, L(ch.ratio[v]), ch.ratio[v].n, ch.ratio[v].blocks, v, xt, c.xs(),
+                                  c.var.ratio, s);
+            launch_batch_col(cg, sg, L(ch.col[v]), ch.col[v].n, ch.col[v].blocks, v, 1, xt, xt, c.xs(), c.var.column,
+                             s);
+            launch_batch_update(rg, sg, L(ch.update[v]), ch.update[v].n, ch.update[v].blocks, v, xe, xt, es, c.xs(),
+                                c.es(), c.var.update, s);
         }
     }
     void enqueue_final(hipStream_t s)
@@ -293,21 +359,20 @@ template <typename S> struct Batch {
                                   c.es(), c.fftW, s);
     }
 
-    void build_graph()
+    void build_graph(Chunk& ch)
     {
-        drop_graph();
         hipStream_t s = cap->s;
         FMVD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            enqueue_iteration(s);
+            enqueue_iteration(ch, s);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(s, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
-        FMVD_HIP(hipStreamEndCapture(s, &graph));
-        FMVD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        FMVD_HIP(hipStreamEndCapture(s, &ch.graph));
+        FMVD_HIP(hipGraphInstantiate(&ch.exec, ch.graph, nullptr, nullptr, 0));
     }
 
     void run(int iterations, int use_graph, hipStream_t s)
@@ -315,11 +380,13 @@ template <typename S> struct Batch {
         if (iterations < 0) throw_msg("fmvd_batch_run: negative iteration count");
         FMVD_HIP(hipSetDevice(device));
         enqueue_init(s);
-        if (use_graph && iterations > 0) {
-            if (!exec) build_graph();
-            for (int it = 0; it < iterations; ++it) FMVD_HIP(hipGraphLaunch(exec, s));
-        } else {
-            for (int it = 0; it < iterations; ++it) enqueue_iteration(s);
+        for (Chunk& ch : chunks) {
+            if (use_graph && iterations > 0) {
+                if (!ch.exec) build_graph(ch);
+                for (int it = 0; it < iterations; ++it) FMVD_HIP(hipGraphLaunch(ch.exec, s));
+            } else {
+                for (int it = 0; it < iterations; ++it) enqueue_iteration(ch, s);
+            }
         }
         enqueue_final(s);
     }
@@ -347,6 +414,8 @@ template <typename S> struct Batch {
                 for (int var = 0; var < nv[kind]; ++var) {
                     *slots[kind] = var;
                     build_lists();
+                    // the class's first chunk, at the size the run launches
+                    const Chunk& ch = chunks[c.chunks.front()];
                     auto launch = [&] {
                         const RowGeom rg = row_geom(c);
                         const ColGeom cg = col_geom(c);
@@ -354,14 +423,14 @@ template <typename S> struct Batch {
                         float2* xt = xe;
                         float* es = c.ES.template as<float>();
                         if (kind == 0)
-                            launch_batch_col(cg, sg, L(c.col[0]), c.col[0].n, c.col[0].blocks, 0, 0, xe, xt, c.xs(),
-                                             var, s);
+                            launch_batch_col(cg, sg, L(ch.col[0]), ch.col[0].n, ch.col[0].blocks, 0, 0, xe, xt,
+                                             c.xs(), var, s);
                         else if (kind == 1)
-                            launch_batch_ratio<S>(rg, sg, L(c.ratio[0]), c.ratio[0].n, c.ratio[0].blocks, 0, xt,
+                            launch_batch_ratio<S>(rg, sg, L(ch.ratio[0]), ch.ratio[0].n, ch.ratio[0].blocks, 0, xt,
                                                   c.xs(), var, s);
                         else
-                            launch_batch_update(rg, sg, L(c.update[0]), c.update[0].n, c.update[0].blocks, 0, xe, xt,
-                                                es, c.xs(), c.es(), var, s);
+                            launch_batch_update(rg, sg, L(ch.update[0]), ch.update[0].n, ch.update[0].blocks, 0, xe,
+                                                xt, es, c.xs(), c.es(), var, s);
                     };
                     launch();
                     FMVD_HIP(hipEventRecord(a, s));

@@ -65,6 +65,25 @@ def test_batch_matches_single_volume_plans(fm, bits):
             assert np.array_equal(v["est"].download(), e), i
 
 
+def test_chunked_batch_matches_single_volume_plans(fm, monkeypatch):
+    """A 2 MiB cache budget splits volumes into plane ranges spread over many
+    chunks (one plane per chunk at 1024 x 1024); results must not change."""
+    monkeypatch.setenv("FMVD_CHUNK_BYTES", str(2 << 20))
+    vols = make_volumes(fm, VOLUMES, seed0=300)
+    iters = 3
+    print(f"\n[batch chunked] {len(vols)} volumes", flush=True)
+    with fm.Batch(vols) as b:
+        b.run(iters, use_graph=True)
+        fm.device_synchronize()
+        graph = [(v["out"].download(), v["est"].download()) for v in vols]
+        b.run(iters, use_graph=False)
+        fm.device_synchronize()
+        for i, v in enumerate(vols):
+            o, e = single(fm, v, iters)
+            assert np.array_equal(graph[i][0], o) and np.array_equal(graph[i][1], e), i
+            assert np.array_equal(v["out"].download(), o) and np.array_equal(v["est"].download(), e), i
+
+
 def test_graph_replays_equal_eager(fm):
     vols = make_volumes(fm, VOLUMES[:4], seed0=200)
     with fm.Batch(vols) as b:
