@@ -8,9 +8,12 @@
  * spectra, weight normalisation, E0 = 10 * nViews and update order), so a
  * batched volume's result equals that plan's fmvd_deconvolve_volume_device_*
  * result for the same FFT plan variants.  Volumes are grouped by padded FFT
- * extent; per RL iteration one launch per (extent class, view index, pass)
- * covers every plane of every volume of the class, and the whole iteration
- * is captured once as a hipGraph and replayed.
+ * extent; each class is cut into cache-resident chunks (whole volumes or
+ * plane ranges of one, so that an iteration's working set fits a 200 MiB
+ * budget, FMVD_CHUNK_BYTES), which run all their iterations in turn.  Per
+ * iteration of a chunk, one launch per (view index, pass) covers every plane
+ * of the chunk, and the iteration is captured once as a hipGraph and
+ * replayed.
  *
  * Errors are reported through the handler installed with setErrorHandler()
  * (fmvd_cuda_utils.h); functions return NULL or a negative value.
@@ -53,8 +56,9 @@ int fmvd_batch_autotune_cuda_8(struct fmvd_batch_cuda_8 *batch, int *chosen);
 
 /* Runs `iterations` RL iterations from E0 = 10 * nViews on every volume and
  * writes d_out / d_est, enqueued on `stream` (hipStream_t; NULL = default
- * stream).  use_graph != 0 replays one captured hipGraph per iteration (the
- * graph is captured on first use and kept until the variants change). */
+ * stream).  use_graph != 0 replays one captured hipGraph per chunk and
+ * iteration (the graphs are captured on first use and kept until the
+ * variants change). */
 int fmvd_batch_run_cuda_16(struct fmvd_batch_cuda_16 *batch, int iterations, int use_graph, void *stream);
 int fmvd_batch_run_cuda_8(struct fmvd_batch_cuda_8 *batch, int iterations, int use_graph, void *stream);
 
