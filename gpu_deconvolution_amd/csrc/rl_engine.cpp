@@ -57,6 +57,7 @@ template <typename S> struct Plan {
     fmvd_estimate_sink_t esink = nullptr;
     void* euser = nullptr;
     int batchCap = 0, chunkCap = 0;
+    int cus = 256;  // compute units of the plan's device
     Variants var;
 
     DeviceBuffer twW, twRow[3], twCol[3];
@@ -137,6 +138,7 @@ template <typename S> struct Plan {
             throw_msg(msg);
         }
         FMVD_HIP(hipGetDevice(&device));
+        FMVD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
         compute.reset(new Stream());
         copy.reset(new Stream());
 
@@ -183,21 +185,30 @@ template <typename S> struct Plan {
     // splits its planes into chunks and runs each chunk's whole iteration
     // chain before the next.  One iteration of a chunk touches its X and ES
     // (19 MB per plane at FFT 1536^2) and samples of every view, plus every
-    // view's two kernel spectra and weights; the chunk is sized so that this
-    // fits a 200 MiB budget in the 256 MB memory-side cache, and rounded down
-    // to a power of two so that launches are whole multiples of the resident
-    // workgroups.  At FFT 1536^2 with 4 views: 4 planes, 1536 row tiles.
+    // view's two kernel spectra and weights; the chunk is the power of two of
+    // planes for which this fits a 200 MiB budget in the 256 MB memory-side
+    // cache (4 planes at FFT 1536^2 with 4 views).  A launch must still fill
+    // the chip: `fill` planes give one residency of the column pass (12
+    // sequences per CU).  When the cache cannot hold that many planes, the
+    // chunk is 4 x fill planes, so that every launch runs several residency
+    // rounds (16 planes at FFT 2560^2).
     int chunk_planes() const
     {
         if (chunkCap > 0) return chunkCap;
         const size_t budget = env_bytes("FMVD_CHUNK_BYTES", (size_t)200 << 20);
         const size_t shared = (size_t)nViews * (2 * xplane() * sizeof(float2) + plane() * sizeof(float));
         const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float) + (size_t)nViews * plane() * sizeof(S);
-        const size_t room = budget > shared ? budget - shared : per;
-        size_t n = std::max<size_t>(1, std::min<size_t>(room / per, 4096));
-        size_t p2 = 1;
-        while (p2 * 2 <= n) p2 *= 2;
-        return (int)p2;
+        const size_t fit = budget > shared ? (budget - shared) / per : 0;
+        auto pow2_floor = [](size_t n) {
+            size_t p = 1;
+            while (p * 2 <= n) p *= 2;
+            return p;
+        };
+        const size_t seqs = (size_t)std::max(1, cus) * 12, want = (seqs + (size_t)N - 1) / (size_t)N;
+        size_t fill = 1;
+        while (fill < want) fill *= 2;
+        const size_t n = fit >= fill ? pow2_floor(std::min<size_t>(fit, 4096)) : std::min<size_t>(4 * fill, 4096);
+        return (int)n;
     }
 
     void run_batch(const S* const* views, long dstride, int np, int iterations, const float* init,
