@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the batched uneven-volume measurement")
     p.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    p.add_argument("--no-full", action="store_true", help="skip the full-size (2048x2048x1024) plain pipeline")
     p.add_argument("--train-samples", type=int, default=2000)
     p.add_argument("--epochs", type=int, default=3)
     p.add_argument("--traffic-run", action="store_true",
@@ -56,7 +57,7 @@ def parse():
     args = p.parse_args()
     if args.traffic_run:
         args.planes, args.iterations, args.steps, args.warmup, args.no_cpu, args.no_batch = 64, 3, 1, 0, True, True
-        args.no_learned = True
+        args.no_learned = args.no_full = True
     return args
 
 
@@ -114,6 +115,64 @@ def batched_workload(fm, synth, seed=2024, nvol=48, iters=25, kernel=31):
                         f"{kernel}x{kernel} PSFs, {iters} iterations as hipGraph replays",
             "value": vvi * iters / el, "unit": "view-voxel RL iterations/s", "seconds": el,
             "classes": classes, "fft_plan_variants": tuned}
+
+
+def full_size_leg(fm, synth, seed=3003, iters=30, kernel=41):
+    """The full-size plain pipeline: 4 uint16 views of 2048 x 2048 x 1024
+    (80k beads, random affines as in the real-size workload), transformed back
+    on the GPU, then `iters` RL iterations with 41 x 41 PSFs (FFT 2560^2).
+    Reports the RL rate and the time to reconstruction (transforms + RL) of
+    one timed run after a one-iteration warm-up.  Returns a dict for the JSON
+    line."""
+    Z, H, W, V = 1024, 2048, 2048, 4
+    shape = (Z, H, W)
+    t0 = time.time()
+    raw = synth.gpu_views(shape, V, beads=80000, seed=seed)
+    rng = np.random.default_rng(seed + 7)
+    A = [synth.random_affine(rng, shape) for _ in range(V)]
+    acquired = []
+    for v in range(V):
+        d = fm.DeviceArray(shape, np.uint16)
+        fm.transform_device(raw[v].ptr, shape, d.ptr, shape, synth.invert_affine(A[v]))
+        acquired.append(d)
+    fm.device_synchronize()
+    for r in raw:
+        r.free()
+    kernels = [synth.gaussian_kernel_2d(*synth.view_sigma((1.5, 1.5, 4.0), v)[:2], kernel) for v in range(V)]
+    weights = []
+    mask = fm.DeviceArray((H, W), np.float32)
+    for v in range(V):
+        fm.mask_device(shape, shape, A[v], 32, 1.0, mask.ptr)
+        weights.append(mask.download())
+    mask.free()
+    registered = [fm.DeviceArray(shape, np.uint16) for _ in range(V)]
+    out = fm.DeviceArray(shape, np.uint16)
+    log(f"[bench] full size: data ready in {time.time() - t0:.1f}s")
+    dec = fm.Deconvolver(H, W, weights, [k.copy() for k in kernels], psf_type=0, bits=16, batch_planes=256)
+    info = dec.info()
+    dec.autotune(planes=info["chunk"])
+    stream = fm.Stream()
+
+    def reconstruct(n_iter):
+        for v in range(V):
+            fm.transform_device(acquired[v].ptr, shape, registered[v].ptr, shape, A[v], stream=stream.handle)
+        dec.run_device([r.ptr for r in registered], Z, n_iter, out_ptr=out.ptr, stream=stream.handle)
+        stream.synchronize()
+
+    reconstruct(1)
+    t = time.perf_counter()
+    reconstruct(iters)
+    el = time.perf_counter() - t
+    rate = V * Z * H * W * iters / el
+    log(f"[bench] full size: {iters} iterations + 4 transforms in {el:.2f}s -> {rate:.4g} view-voxel it/s "
+        f"(plan {info})")
+    dec.close()
+    for d in acquired + registered + [out]:
+        d.free()
+    return {"workload": f"4 views of 2048x2048x1024 uint16, 80k beads, random affines, {kernel}x{kernel} PSFs, "
+                        f"transform back + {iters} plain RL iterations (FFT {info['fft_h']}x{info['fft_w']})",
+            "value": rate, "unit": "view-voxel RL iterations/s", "seconds_to_reconstruction": el,
+            "planes_per_launch": info["chunk"]}
 
 
 def learned_leg(fm, oracle, ntrain=2000, epochs=3, label_iters=50, ntest=2, lr=2e-4, seed=7, tol=1e-3):
@@ -324,12 +383,16 @@ def main():
                          f"(FFT {fh}x{fw}), OpenMP over planes; plan setup ({t_setup:.1f}s) subtracted"}
         log(f"[bench] cpu baseline {cpu['value']:.4g} view-voxel it/s on {nt} threads ({t_iter:.1f}s)")
 
+    for r in registered + acquired + [out]:
+        r.free()
+    dec.close()
     batched = None
     if not args.no_batch:
-        for r in registered + acquired + [out]:
-            r.free()
-        dec.close()
         batched = batched_workload(fm, synth)
+
+    full = None
+    if not args.no_full:
+        full = full_size_leg(fm, synth)
 
     learned = None
     if not args.no_learned and not args.no_cpu:
@@ -372,6 +435,7 @@ def main():
         "transform_gbs": tr_gbs,
         "cpu_baseline": cpu,
         "batched": batched,
+        "full_size": full,
         "learned_vs_plain": learned,
     }
     print(json.dumps(line), flush=True)

@@ -23,10 +23,12 @@ struct Bead {
     float cz, cy, cx, sigma, amp;
 };
 
+// Element-wise kernels walk the volume grid-stride: a launch may hold at
+// most 2^32 work-items per grid dimension, below a 2048 x 2048 x 1024 volume.
 __global__ void fill_kernel(float* __restrict__ v, long n, float value)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] = value;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        v[i] = value;
 }
 
 __global__ void beads_kernel(float* __restrict__ vol, int Z, int H, int W, const Bead* __restrict__ beads)
@@ -58,8 +60,7 @@ __global__ void blur_kernel(const float* __restrict__ in, float* __restrict__ ou
                             Taps t)
 {
     const long n = (long)Z * H * W;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const int x = (int)(i % W);
     const int y = (int)((i / W) % H);
     const int z = (int)(i / ((long)W * H));
@@ -74,6 +75,7 @@ __global__ void blur_kernel(const float* __restrict__ in, float* __restrict__ ou
         acc += t.g[k + t.r] * in[base + (long)cc * stride];
     }
     out[i] = acc;
+    }
 }
 
 __device__ __forceinline__ uint64_t splitmix(uint64_t& s)
@@ -118,12 +120,12 @@ __device__ double poisson(double lam, uint64_t& s)
 template <typename S>
 __global__ void poisson_kernel(const float* __restrict__ lam, S* __restrict__ out, long n, uint64_t seed, double maxv)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t s = seed ^ ((uint64_t)i * 0xD1342543DE82EF95ull);
-    (void)splitmix(s);
-    double k = poisson((double)lam[i], s);
-    out[i] = (S)(k > maxv ? maxv : k);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        uint64_t s = seed ^ ((uint64_t)i * 0xD1342543DE82EF95ull);
+        (void)splitmix(s);
+        const double k = poisson((double)lam[i], s);
+        out[i] = (S)(k > maxv ? maxv : k);
+    }
 }
 
 // Plain streaming copy (16 bytes per thread): the known-size reference kernel
@@ -135,6 +137,12 @@ __global__ void copy_kernel(const uint4* __restrict__ in, uint4* __restrict__ ou
 }
 
 inline unsigned blocks(long n, int t) { return (unsigned)((n + t - 1) / t); }
+// grid of the grid-stride kernels
+inline unsigned strided_blocks(long n, int t)
+{
+    const long b = (n + t - 1) / t;
+    return (unsigned)(b < (1L << 20) ? b : (1L << 20));
+}
 
 }  // namespace
 
@@ -146,7 +154,7 @@ int fmvd_synth_truth(float* vol, int Z, int H, int W, const float* beads5, int n
     FMVD_API_GUARD(-1, {
         hipStream_t s = (hipStream_t)stream;
         const long n = (long)Z * H * W;
-        fill_kernel<<<blocks(n, 256), 256, 0, s>>>(vol, n, background);
+        fill_kernel<<<strided_blocks(n, 256), 256, 0, s>>>(vol, n, background);
         FMVD_LAUNCH_CHECK();
         if (nbeads > 0) {
             Bead* d = nullptr;
@@ -173,7 +181,7 @@ int fmvd_synth_blur(const float* in, float* out, int Z, int H, int W, int axis, 
         for (int k = -t.r; k <= t.r; ++k) sum += g[k + t.r] = std::exp(-0.5 * (k / (double)sigma) * (k / (double)sigma));
         for (int k = 0; k <= 2 * t.r; ++k) t.g[k] = (float)(g[k] / sum);
         const long n = (long)Z * H * W;
-        blur_kernel<<<blocks(n, 256), 256, 0, (hipStream_t)stream>>>(in, out, Z, H, W, axis, t);
+        blur_kernel<<<strided_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(in, out, Z, H, W, axis, t);
         FMVD_LAUNCH_CHECK();
         return 0;
     })
@@ -184,9 +192,9 @@ int fmvd_synth_poisson(const float* lam, void* out, long n, unsigned long long s
     FMVD_API_GUARD(-1, {
         hipStream_t s = (hipStream_t)stream;
         if (bits == 16)
-            poisson_kernel<uint16_t><<<blocks(n, 256), 256, 0, s>>>(lam, (uint16_t*)out, n, seed, 65535.0);
+            poisson_kernel<uint16_t><<<strided_blocks(n, 256), 256, 0, s>>>(lam, (uint16_t*)out, n, seed, 65535.0);
         else
-            poisson_kernel<uint8_t><<<blocks(n, 256), 256, 0, s>>>(lam, (uint8_t*)out, n, seed, 255.0);
+            poisson_kernel<uint8_t><<<strided_blocks(n, 256), 256, 0, s>>>(lam, (uint8_t*)out, n, seed, 255.0);
         FMVD_LAUNCH_CHECK();
         return 0;
     })
