@@ -322,11 +322,12 @@ FMVD_PLAN_VARIANT_GW(768, 1, 64, 4, 5, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT_GW(768, 2, 64, 4, 6, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
-FMVD_PLAN_VARIANT(1280, 1, 32, 8, 8, 20)
+FMVD_PLAN_VARIANT_GW(1280, 1, 64, 4, 4, 4, 4, 4, 20)
 FMVD_PLAN_VARIANT(1280, 2, 64, 4, 4, 4, 4, 5)
 FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
 FMVD_PLAN_VARIANT(1536, 2, 128, 4, 4, 4, 4, 6)
 FMVD_PLAN_VARIANT(2560, 1, 64, 4, 8, 8, 10)
+FMVD_PLAN_VARIANT(2560, 2, 128, 4, 4, 4, 4, 10)
 #undef FMVD_PLAN_VARIANT
 #undef FMVD_PLAN_VARIANT_G
 #undef FMVD_PLAN_VARIANT_GW
