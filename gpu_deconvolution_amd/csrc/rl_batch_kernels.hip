@@ -54,7 +54,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void batch_ratio_kern
     const int local = lid - list[n + e], nrg = g.H / C::G;
     const int rg = local % nrg, pl = local / nrg;
     const SAMPLE* d = static_cast<const SAMPLE*>(S.data[view]) + (long)pl * S.dstride;
-    ratio_tile<SAMPLE, PL>(seg_geom(g, S), XT + (long)(S.plane0 + pl) * xstride, d, rg * C::G, lds);
+    ratio_tile<SAMPLE, PL>(seg_geom(g, S), XT + (long)(S.plane0 + pl) * xstride, d, rg * C::G, lds, 1.0f);
 }
 
 template <typename PL>

@@ -12,6 +12,9 @@
 // split with A[k] = (C[k] + conj C[-k])/2, B[k] = (C[k] - conj C[-k])/(2i),
 // multiplied by the DC and Nyquist kernel columns separately and recombined
 // as A*K0 + i*B*KN, whose inverse is again DC + i*Nyquist of the result.
+//
+// Every plan is compiled twice: multiplying by K, and by conj(K) for pass C
+// when the kernel-hat spectrum is the conjugate of the kernel's.
 #include "fft_device.hpp"
 #include "hip_check.hpp"
 #include "rl_launch.hpp"
@@ -20,7 +23,7 @@
 
 namespace fmvd {
 
-template <typename PL>
+template <typename PL, bool CONJ>
 __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(ColGeom g, const float2* Xin,
                                                                        float2* Xout, long xstride,
                                                                        const float2* __restrict__ K, int nplanes)
@@ -30,7 +33,7 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     // plane-fastest: consecutive logical blocks reuse the same kernel rows
     const int p = lid % nplanes, grp = lid / nplanes;
-    col_tile<PL>(g, Xin + p * xstride, Xout + p * xstride, K, grp, lds);
+    col_tile<PL, CONJ>(g, Xin + p * xstride, Xout + p * xstride, K, grp, lds);
 }
 
 #define FMVD_COL_SIZES(X) \
@@ -66,24 +69,25 @@ int col_variants(int H)
 
 template <typename PL>
 static void launch_col_plan(const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K,
-                            int nplanes, hipStream_t s)
+                            int nplanes, hipStream_t s, bool conj)
 {
     using C = ColCfg<PL>;
     if (g.N % C::SQ) throw_msg("row count not divisible by column group");
     dim3 grid((unsigned)(g.N / C::SQ * nplanes));
-    launch(col_conv_kernel<PL>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes);
+    if (conj) launch(col_conv_kernel<PL, true>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes);
+    else launch(col_conv_kernel<PL, false>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes);
 }
 
 template <int H>
 static void col_variant(int v, const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K,
-                        int nplanes, hipStream_t s)
+                        int nplanes, hipStream_t s, bool conj)
 {
     if (v == 1 && PlanV<H, 1>::distinct) {
-        if constexpr (PlanV<H, 1>::distinct) launch_col_plan<PlanV<H, 1>>(g, Xin, Xout, xstride, K, nplanes, s);
+        if constexpr (PlanV<H, 1>::distinct) launch_col_plan<PlanV<H, 1>>(g, Xin, Xout, xstride, K, nplanes, s, conj);
     } else if (v == 2 && PlanV<H, 2>::distinct) {
-        if constexpr (PlanV<H, 2>::distinct) launch_col_plan<PlanV<H, 2>>(g, Xin, Xout, xstride, K, nplanes, s);
+        if constexpr (PlanV<H, 2>::distinct) launch_col_plan<PlanV<H, 2>>(g, Xin, Xout, xstride, K, nplanes, s, conj);
     } else {
-        launch_col_plan<PlanV<H, 0>>(g, Xin, Xout, xstride, K, nplanes, s);
+        launch_col_plan<PlanV<H, 0>>(g, Xin, Xout, xstride, K, nplanes, s, conj);
     }
 }
 
@@ -107,12 +111,12 @@ std::vector<float2> col_pass_twiddles(int H, int variant)
 }
 
 void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride, const float2* K, int nplanes,
-                     int variant, hipStream_t s)
+                     int variant, hipStream_t s, bool conj)
 {
     switch (g.H) {
 #define FMVD_CASE(n)                                                          \
     case n:                                                                   \
-        col_variant<n>(variant, g, Xin, Xout, xstride, K, nplanes, s);        \
+        col_variant<n>(variant, g, Xin, Xout, xstride, K, nplanes, s, conj);  \
         break;
         FMVD_COL_SIZES(FMVD_CASE)
 #undef FMVD_CASE

@@ -44,6 +44,12 @@ static size_t env_bytes(const char* name, size_t dflt)
     return (size_t)strtoull(s, nullptr, 10);
 }
 
+static bool env_flag(const char* name)
+{
+    const char* s = getenv(name);
+    return s && *s && strcmp(s, "0") != 0;
+}
+
 // ------------------------------------------------------------------ plan
 template <typename S> struct Plan {
     using source_t = int (*)(S**, int);
@@ -62,10 +68,17 @@ template <typename S> struct Plan {
 
     DeviceBuffer twW, twRow[3], twCol[3];
     std::vector<DeviceBuffer> K, Kh, Wn;
+    // Kernel-hat spectrum = conj(kernel spectrum): independent PSFs (the hat is
+    // the kernel flipped about its centre) of odd width and height.  Pass C
+    // then reads K with conjugation and Kh is not built; the ratio pass folds
+    // in the 1/W that Kh would carry.
+    bool khatConj = false;
     DeviceBuffer XE, ES;  // XE: the spectrum buffer X of every pass
     std::unique_ptr<Stream> compute, copy;
 
     size_t plane() const { return (size_t)dataH * dataW; }
+    const float2* khat(int v) const { return (khatConj ? K[v] : Kh[v]).as<float2>(); }
+    float ratio_scale() const { return khatConj ? 1.0f / (float)fftW : 1.0f; }
     size_t xplane() const { return (size_t)N * fftH; }  // complex per plane
     size_t eplane() const { return (size_t)fftW * fftH; }  // floats per plane
 
@@ -146,6 +159,7 @@ template <typename S> struct Plan {
         // then the caller's kernels are normalised in place.
         const int len = kernelH * kernelW;
         std::vector<double> khat = compute_kernel_hat(h_Kernel, kernelW, kernelH, nViews, psf);
+        khatConj = psf == 0 && (kernelW % 2) == 1 && (kernelH % 2) == 1 && !env_flag("FMVD_EXPLICIT_KHAT");
         for (int v = 0; v < nViews; ++v) normalize_kernel_f32(h_Kernel[v], len);
 
         upload(twW, twiddles(fftW, N));
@@ -164,8 +178,9 @@ template <typename S> struct Plan {
             // pass A consumes XE = R2C(E)/W: fold 1/H here; pass C consumes
             // R2C(R): fold 1/(H W) (modulateAndNormalize's c, fmvd_deconvolve_cuda.cu:369)
             upload(K[v], kernel_spectrum_transposed(kv.data(), kernelW, kernelH, fftH, fftW, 1.0 / fftH));
-            upload(Kh[v], kernel_spectrum_transposed(khat.data() + (size_t)v * len, kernelW, kernelH,
-                                                     fftH, fftW, 1.0 / ((double)fftH * fftW)));
+            if (!khatConj)
+                upload(Kh[v], kernel_spectrum_transposed(khat.data() + (size_t)v * len, kernelW, kernelH,
+                                                         fftH, fftW, 1.0 / ((double)fftH * fftW)));
         }
         std::vector<float> wn = normalized_weights(h_Weights, dataW, dataH, nViews);
         for (int v = 0; v < nViews; ++v) {
@@ -185,7 +200,8 @@ template <typename S> struct Plan {
     // splits its planes into chunks and runs each chunk's whole iteration
     // chain before the next.  One iteration of a chunk touches its X and ES
     // (19 MB per plane at FFT 1536^2) and samples of every view, plus every
-    // view's two kernel spectra and weights; the chunk is the power of two of
+    // view's kernel spectra (one when the hat is the conjugate, else two) and
+    // weights; the chunk is the power of two of
     // planes for which this fits a 200 MiB budget in the 256 MB memory-side
     // cache (4 planes at FFT 1536^2 with 4 views).  A launch must still fill
     // the chip: `fill` planes give one residency of the column pass (12
@@ -196,7 +212,8 @@ template <typename S> struct Plan {
     {
         if (chunkCap > 0) return chunkCap;
         const size_t budget = env_bytes("FMVD_CHUNK_BYTES", (size_t)200 << 20);
-        const size_t shared = (size_t)nViews * (2 * xplane() * sizeof(float2) + plane() * sizeof(float));
+        const size_t spectra = khatConj ? 1 : 2;
+        const size_t shared = (size_t)nViews * (spectra * xplane() * sizeof(float2) + plane() * sizeof(float));
         const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float) + (size_t)nViews * plane() * sizeof(S);
         const size_t fit = budget > shared ? (budget - shared) / per : 0;
         auto pow2_floor = [](size_t n) {
@@ -253,11 +270,11 @@ template <typename S> struct Plan {
                 }
                 {
                     TimedScope ts(kTimeRatio, s, timed);
-                    launch_row_ratio<S>(rg, xt, xs, dv, dstride, np, var.ratio, s);
+                    launch_row_ratio<S>(rg, xt, xs, dv, dstride, np, var.ratio, s, ratio_scale());
                 }
                 {
                     TimedScope ts(kTimeColumn, s, timed);
-                    launch_col_conv(cg, xt, xt, xs, Kh[v].as<float2>(), np, var.column, s);
+                    launch_col_conv(cg, xt, xt, xs, khat(v), np, var.column, s, khatConj);
                 }
                 {
                     TimedScope ts(kTimeUpdate, s, timed);
@@ -320,8 +337,8 @@ template <typename S> struct Plan {
                 float ms = -1.f;
                 if (v < nv) {
                     switch (k) {
-                    case 0: ms = time_it([&] { launch_col_conv(cg, xt, xt, xs, Kh[0].as<float2>(), planes, v, s); }); break;
-                    case 1: ms = time_it([&] { launch_row_ratio<S>(rg, xt, xs, data.as<S>(), (long)pl, planes, v, s); }); break;
+                    case 0: ms = time_it([&] { launch_col_conv(cg, xt, xt, xs, khat(0), planes, v, s, khatConj); }); break;
+                    case 1: ms = time_it([&] { launch_row_ratio<S>(rg, xt, xs, data.as<S>(), (long)pl, planes, v, s, ratio_scale()); }); break;
                     case 2: ms = time_it([&] { launch_row_update(rg, xe, xt, es, xs, esd, Wn[0].as<float>(), planes, v, s); }); break;
                     case 3: ms = time_it([&] { launch_row_init(rg, xt, es, xs, esd, e0.as<float>(), (long)pl, planes, v, s); }); break;
                     case 4: ms = time_it([&] { launch_final<S>(rg, es, esd, out.as<S>(), nullptr, (long)pl, planes, s); }); break;

@@ -41,11 +41,11 @@ struct ColGeom {
     const float2* twP[3]; // per-pass twiddle tables of the column FFT plan variants 0..2
 };
 
-// Pass B: X <- R2C(ratio(D, C2R(X))) for nplanes planes, in place.
+// Pass B: X <- R2C(ratio(D, C2R(X))) * scale for nplanes planes, in place.
 // data: first plane of this view; dstride: elements between planes.
 template <typename SAMPLE>
 void launch_row_ratio(const RowGeom& g, float2* X, long xstride, const SAMPLE* data, long dstride,
-                      int nplanes, int variant, hipStream_t s);
+                      int nplanes, int variant, hipStream_t s, float scale = 1.0f);
 // Pass D: C = C2R(XC); E += w*(E*C - E) on the spatial estimate ES (fftH x
 // fftW floats per plane, row-major); XE <- R2C(E)/W.
 void launch_row_update(const RowGeom& g, float2* XE, const float2* XC, float* ES, long xstride, long estride,
@@ -60,9 +60,10 @@ void launch_row_init(const RowGeom& g, float2* XE, float* ES, long xstride, long
 // Constant start E0 = value everywhere: ES = value, XE[0][y] = (value, 0), rest 0.
 void launch_const_init(const RowGeom& g, float2* XE, float* ES, long xstride, long estride, float value,
                        int nplanes, hipStream_t s);
-// Passes A / C: Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]) for every kx.
+// Passes A / C: Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]) for every kx
+// (conj(K[kx]) when conj is set).
 void launch_col_conv(const ColGeom& g, const float2* Xin, float2* Xout, long xstride,
-                     const float2* K, int nplanes, int variant, hipStream_t s);
+                     const float2* K, int nplanes, int variant, hipStream_t s, bool conj = false);
 
 // Per-pass twiddle tables (host side) of a row / column FFT plan variant.
 std::vector<float2> row_pass_twiddles(int N, int variant);

@@ -24,14 +24,15 @@ namespace fmvd {
 // ------------------------------------------------------------------ pass B
 template <typename SAMPLE, typename PL>
 __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_ratio_kernel(
-    RowGeom g, float2* __restrict__ X, long xstride, const SAMPLE* __restrict__ data, long dstride, int nplanes)
+    RowGeom g, float2* __restrict__ X, long xstride, const SAMPLE* __restrict__ data, long dstride, int nplanes,
+    float scale)
 {
     using C = RowCfg<PL>;
     __shared__ float2 lds[C::G * C::L];
     const int nrg = g.H / C::G;
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     const int rg = lid % nrg, p = lid / nrg;
-    ratio_tile<SAMPLE, PL>(g, X + p * xstride, data + p * dstride, rg * C::G, lds);
+    ratio_tile<SAMPLE, PL>(g, X + p * xstride, data + p * dstride, rg * C::G, lds, scale);
 }
 
 // ------------------------------------------------------------------ pass D
@@ -223,11 +224,11 @@ std::vector<float2> row_pass_twiddles(int N, int variant)
 
 template <typename SAMPLE>
 void launch_row_ratio(const RowGeom& g, float2* X, long xstride, const SAMPLE* data, long dstride, int nplanes,
-                      int variant, hipStream_t s)
+                      int variant, hipStream_t s, float scale)
 {
     dispatch_row(g.N, variant, [&]<typename PL>() {
         launch(row_ratio_kernel<SAMPLE, PL>, row_grid<PL>(g.H, nplanes), dim3(RowCfg<PL>::THREADS), 0, s, g, X,
-               xstride, data, dstride, nplanes);
+               xstride, data, dstride, nplanes, scale);
     });
     FMVD_LAUNCH_CHECK();
 }
@@ -272,9 +273,9 @@ void launch_const_init(const RowGeom& g, float2* XE, float* ES, long xstride, lo
 }
 
 template void launch_row_ratio<uint16_t>(const RowGeom&, float2*, long, const uint16_t*, long, int, int,
-                                         hipStream_t);
+                                         hipStream_t, float);
 template void launch_row_ratio<uint8_t>(const RowGeom&, float2*, long, const uint8_t*, long, int, int,
-                                        hipStream_t);
+                                        hipStream_t, float);
 template void launch_final<uint16_t>(const RowGeom&, const float*, long, uint16_t*, float*, long, int,
                                      hipStream_t);
 template void launch_final<uint8_t>(const RowGeom&, const float*, long, uint8_t*, float*, long, int, hipStream_t);

@@ -161,12 +161,13 @@ __device__ __forceinline__ void c2r_pre(float2* v, const float2* row, int t, con
 // R2C post-processing and transposed store: v holds Z (forward FFT of the
 // packed row) at the forward's exit layout.  X[k] = (Z[k]+conj Z[N-k])/2 +
 // exp(-2 pi i k/W) (Z[k]-conj Z[N-k])/(2i);  X[0] = (ReZ0+ImZ0, ReZ0-ImZ0).
-__device__ __forceinline__ float2 r2c_combine(float2 zk, float2 zmc, float2 w)
+// h = scale / 2: the output scale rides on the halving (no extra multiplies)
+__device__ __forceinline__ float2 r2c_combine(float2 zk, float2 zmc, float2 w, float h)
 {
     const float2 zm = cconj(zmc);
-    const float2 fe = cscale(cadd(zk, zm), 0.5f);
+    const float2 fe = cscale(cadd(zk, zm), h);
     const float2 d = csub(zk, zm);
-    const float2 fo = make_float2(0.5f * d.y, -0.5f * d.x);
+    const float2 fo = make_float2(h * d.y, -h * d.x);
     return cadd(fe, cmul(w, fo));
 }
 
@@ -176,6 +177,7 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
                                                float scale)
 {
     using PL = typename C::PL;
+    const float h = 0.5f * scale;
     float2* row = lds + seq * C::L;
     if constexpr (affine_layout<C, C::RLF>()) {
         constexpr int M = C::N / C::RLF;
@@ -203,12 +205,12 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
             float2 xo;
             if (k == 0) {
                 const int im = kx0 == 0 ? pk : C::N + C::N / 16 + mk;
-                const float2 z = r2c_combine(zk, r[im], twW[kx0]);
-                xo = kx0 == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y) : z;
+                const float2 z = r2c_combine(zk, r[im], twW[kx0], h);
+                xo = kx0 == 0 ? make_float2((zk.x + zk.y) * scale, (zk.x - zk.y) * scale) : z;
             } else {
-                xo = r2c_combine(zk, r[(C::N - c) + (C::N - c) / 16 + mk], twW[kx0 + c]);
+                xo = r2c_combine(zk, r[(C::N - c) + (C::N - c) / 16 + mk], twW[kx0 + c], h);
             }
-            X[off0 + k * step] = cscale(xo, scale);
+            X[off0 + k * step] = xo;
         }
     } else {
 #pragma unroll 4
@@ -217,9 +219,9 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
             tile_elem<C>(tid, k, g, kx);
             const float2* r = lds + g * C::L;
             const float2 zk = r[lds_pad(kx)];
-            const float2 xo = kx == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y)
-                                      : r2c_combine(zk, r[lds_pad(C::N - kx)], twW[kx]);
-            X[(unsigned)xpos(kx, y0 + g, H)] = cscale(xo, scale);
+            const float2 xo = kx == 0 ? make_float2((zk.x + zk.y) * scale, (zk.x - zk.y) * scale)
+                                      : r2c_combine(zk, r[lds_pad(C::N - kx)], twW[kx], h);
+            X[(unsigned)xpos(kx, y0 + g, H)] = xo;
         }
     }
 }
@@ -281,12 +283,13 @@ __device__ __forceinline__ void row_c2r(float2* v, float2* row, int t, const Row
 // G*RowCfg::L complex.  g carries the FFT extent (uniform over the launch)
 // and the data extent and pad half-widths of the plane's volume.
 
-// Pass B: X <- R2C(Q > 1 ? D / Q : Q), Q = C2R(X).  divide_kernel_{16,8},
+// Pass B: X <- R2C(Q > 1 ? D / Q : Q) * scale, Q = C2R(X).  divide_kernel_{16,8},
 // fmvd_deconvolve_cuda.impl.cu:159-176; the data is read unpadded through the
-// data pad map (swapped half-widths, fmvd_deconvolve.impl.cpp:36-37).
+// data pad map (swapped half-widths, fmvd_deconvolve.impl.cpp:36-37).  scale
+// is 1, or 1/W when pass C's kernel spectrum is pass A's (which carries 1/H only).
 template <typename SAMPLE, typename PL>
 __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict__ Xp,
-                                           const SAMPLE* __restrict__ dplane, int y0, float2* lds)
+                                           const SAMPLE* __restrict__ dplane, int y0, float2* lds, float scale)
 {
     using C = RowCfg<PL>;
     using PairT = typename PairOf<SAMPLE>::type;
@@ -318,7 +321,7 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
         v[i].y = q1 > 1.0f ? d1 * __builtin_amdgcn_rcpf(q1) : q1;
     }
     fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
-    r2c_post_store<C>(v, lds, Xp, g.H, y0, tid, seq, t, g.twW, 1.0f);
+    r2c_post_store<C>(v, lds, Xp, g.H, y0, tid, seq, t, g.twW, scale);
 }
 
 // Pass D: C = C2R(XC); E <- E + w (E C - E) on the spatial estimate (one
@@ -413,6 +416,19 @@ template <typename PL_> struct ColCfg {
     static_assert(RF == last_radix<PL, +1>(), "inverse exit layout must match forward entry");
 };
 
+// v[i] *= Kr[y] (conj(Kr[y]) for CONJ) at the inverse-entry positions y of
+// thread t; Kr is one kx-row of a kernel spectrum (stride 2 along y).
+template <typename PL, bool CONJ>
+__device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ Kr, int t)
+{
+    constexpr int RL = last_radix<PL, -1>();
+#pragma unroll
+    for (int i = 0; i < PL::P; ++i) {
+        const float2 k = Kr[2 * slot_pos<PL, RL>(t, i)];
+        v[i] = CONJ ? cmulc(v[i], k) : cmul(v[i], k);
+    }
+}
+
 // Passes A / C on the kx-rows [grp*SQ, grp*SQ+SQ) of one plane:
 // Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]); Xout may be Xin (each wave holds
 // its whole sequence in registers before it stores).  The cuFFT column halves plus
@@ -421,7 +437,10 @@ template <typename PL_> struct ColCfg {
 // DC + i*Nyquist: its spectrum is split with A[k] = (C[k] + conj C[-k])/2,
 // B[k] = (C[k] - conj C[-k])/(2i), multiplied by the DC and Nyquist kernel
 // columns and recombined as A*K0 + i*B*KN.
-template <typename PL>
+// CONJ: multiply by conj(K) instead.  The spectrum of a real kernel flipped
+// about its centre is the conjugate of the kernel's spectrum, so pass C can
+// read pass A's K.
+template <typename PL, bool CONJ = false>
 __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, float2* Xout,
                                          const float2* __restrict__ K, int grp, float2* lds)
 {
@@ -456,20 +475,17 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
                 const float2 a = cscale(cadd(ck, cm), 0.5f);
                 const float2 d = csub(ck, cm);
                 const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
-                const float2 ak = cmul(a, K0[2 * k]);
-                const float2 bk = cmul(b, KN[k]);
+                const float2 k0 = K0[2 * k], kn = KN[k];
+                const float2 ak = CONJ ? cmulc(a, k0) : cmul(a, k0);
+                const float2 bk = CONJ ? cmulc(b, kn) : cmul(b, kn);
                 v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
             }
         } else {
-            const float2* Kr = K + (unsigned)xpos(kx, 0, H);
-#pragma unroll
-            for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[2 * slot_pos<PL, C::RL>(t, i)]);
+            col_kmul<PL, CONJ>(v, K + (unsigned)xpos(kx, 0, H), t);
         }
         seq_sync<PL>();
     } else {
-        const float2* Kr = K + (unsigned)xpos(kx, 0, H);
-#pragma unroll
-        for (int i = 0; i < C::P; ++i) v[i] = cmul(v[i], Kr[2 * slot_pos<PL, C::RL>(t, i)]);
+        col_kmul<PL, CONJ>(v, K + (unsigned)xpos(kx, 0, H), t);
     }
 
     fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);

@@ -45,6 +45,7 @@ CASES = [
     (1, 2, 50, 3200, 41, 41, 0, 16, 2),       # 128 x 3584 (row length 1792)
     (1, 2, 50, 1200, 41, 41, 0, 16, 2),       # 128 x 1536 (row length 768)
     (1, 2, 50, 2300, 41, 41, 0, 16, 2),       # 128 x 2560 (row length 1280)
+    (2, 2, 100, 90, 16, 12, 0, 16, 5),        # 128 x 128, even kernel: explicit kernel-hat spectrum
 ]
 
 
@@ -93,6 +94,26 @@ def test_every_fft_plan_variant_matches_oracle(fm, orc, case):
         dec.set_variants([c for c, _ in tuned.values()])
         out, est = dec.run(planes, 2, want_estimate=True)
         check_rl(out, est, out_o, est_o)
+
+
+@pytest.mark.parametrize("dh, dw, k", [(64, 80, 15), (1024, 1024, 41)])
+def test_conjugate_kernel_hat_matches_explicit(fm, orc, monkeypatch, dh, dw, k):
+    """Independent PSFs of odd size: pass C multiplies by the conjugate of the
+    kernel spectrum instead of a separate kernel-hat spectrum (the flipped
+    kernel's).  FMVD_EXPLICIT_KHAT=1 builds the explicit one; both agree with
+    each other to FP32 rounding and with the oracle."""
+    Z, V = 2, 3
+    data, kernels, weights = make_case(Z, V, dh, dw, k, k, 16, seed=dh + k)
+    monkeypatch.delenv("FMVD_EXPLICIT_KHAT", raising=False)
+    out_c, est_c, _ = run_gpu(fm, data, kernels, weights, 0, 4, 16)
+    monkeypatch.setenv("FMVD_EXPLICIT_KHAT", "1")
+    out_e, est_e, _ = run_gpu(fm, data, kernels, weights, 0, 4, 16)
+    rel = float(np.linalg.norm(est_c - est_e) / np.linalg.norm(est_e))
+    print(f"\n[khat] {dh}x{dw} k={k}: conj vs explicit relL2={rel:.2e}", flush=True)
+    assert rel < 1e-5
+    out_o, est_o = orc.rl_planes(data, weights, kernels.copy(), 0, 4, threads=8)
+    check_rl(out_c, est_c, out_o, est_o)
+    check_rl(out_e, est_e, out_o, est_o)
 
 
 def test_batches_and_device_volume_agree(fm):
