@@ -21,7 +21,9 @@ struct BatchSeg {
     const void* data[kBatchMaxViews];   // views, plane-major SAMPLE volumes
     const float* w[kBatchMaxViews];     // normalised weights, dataH x dataW
     const float2* K[kBatchMaxViews];    // kernel spectra (transposed layout, pass A scaling)
-    const float2* Kh[kBatchMaxViews];   // kernel-hat spectra (pass C scaling)
+    const float2* Kh[kBatchMaxViews];   // kernel-hat spectra (pass C scaling), or K when khconj
+    int khconj;              // pass C multiplies by conj(Kh): the hat is the flipped kernel
+    float rscale;            // ratio-pass output scale: 1, or 1/fftW when khconj
     void* out;               // SAMPLE result volume or null
     float* est;              // float estimate volume or null
 };

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <vector>
@@ -41,6 +42,8 @@ template <typename S> struct Batch {
     struct Vol {
         int dataH = 0, dataW = 0, nplanes = 0, nViews = 0, kH = 0, kW = 0;
         int cls = 0, plane0 = 0;
+        bool khconj = false;  // kernel-hat spectrum = conj(K) (odd independent PSFs): Kh not built
+        float rscale = 1.0f;
         std::vector<DeviceBuffer> K, Kh, Wn;
         const S* views[kBatchMaxViews] = {};
         S* out = nullptr;
@@ -88,6 +91,12 @@ template <typename S> struct Batch {
         }
     }
 
+    static bool explicit_khat()
+    {
+        const char* e = getenv("FMVD_EXPLICIT_KHAT");
+        return e && *e && strcmp(e, "0") != 0;
+    }
+
     static size_t chunk_budget()
     {
         const char* e = getenv("FMVD_CHUNK_BYTES");
@@ -115,8 +124,10 @@ template <typename S> struct Batch {
             s.data[k] = v.views[k];
             s.w[k] = v.Wn[k].template as<float>();
             s.K[k] = v.K[k].template as<float2>();
-            s.Kh[k] = v.Kh[k].template as<float2>();
+            s.Kh[k] = (v.khconj ? v.K[k] : v.Kh[k]).template as<float2>();
         }
+        s.khconj = v.khconj ? 1 : 0;
+        s.rscale = v.rscale;
         s.out = v.out;
         s.est = v.est;
         return s;
@@ -187,12 +198,16 @@ template <typename S> struct Batch {
             v.K.resize(a.nViews);
             v.Kh.resize(a.nViews);
             v.Wn.resize(a.nViews);
+            // as Plan::init: an odd independent PSF's hat spectrum is conj(K)
+            v.khconj = a.psf_type == 0 && a.kernelW % 2 == 1 && a.kernelH % 2 == 1 && !explicit_khat();
+            v.rscale = v.khconj ? 1.0f / (float)fftW : 1.0f;
             std::vector<double> kv(len);
             for (int k = 0; k < a.nViews; ++k) {
                 for (int j = 0; j < len; ++j) kv[j] = (double)a.h_kernels[k][j];
                 upload(v.K[k], kernel_spectrum_transposed(kv.data(), a.kernelW, a.kernelH, fftH, fftW, 1.0 / fftH));
-                upload(v.Kh[k], kernel_spectrum_transposed(khat.data() + (size_t)k * len, a.kernelW, a.kernelH, fftH,
-                                                           fftW, 1.0 / ((double)fftH * fftW)));
+                if (!v.khconj)
+                    upload(v.Kh[k], kernel_spectrum_transposed(khat.data() + (size_t)k * len, a.kernelW, a.kernelH,
+                                                               fftH, fftW, 1.0 / ((double)fftH * fftW)));
             }
             const size_t plane = (size_t)a.dataH * a.dataW;
             std::vector<float> wn = normalized_weights(a.h_weights, a.dataW, a.dataH, a.nViews);
@@ -224,11 +239,12 @@ template <typename S> struct Batch {
             for (int i : c.vols) {
                 const Vol& v = vols[i];
                 const size_t plane = (size_t)v.dataH * v.dataW;
-                // per plane: spectrum, estimate, every view's samples; per
-                // volume: every view's two kernel spectra and weights
-                const size_t pp = (size_t)c.xs() * sizeof(float2) + (size_t)c.es() * sizeof(float) +
-                                  (size_t)v.nViews * plane * sizeof(S);
-                const size_t pv = (size_t)v.nViews * (2 * (size_t)c.xs() * sizeof(float2) + plane * sizeof(float));
+                // per plane: spectrum, estimate (the samples are streamed with
+                // non-temporal loads); per volume: every view's kernel spectra
+                // (one or two) and weights
+                const size_t pp = (size_t)c.xs() * sizeof(float2) + (size_t)c.es() * sizeof(float);
+                const size_t nsp = v.khconj ? 1 : 2;
+                const size_t pv = (size_t)v.nViews * (nsp * (size_t)c.xs() * sizeof(float2) + plane * sizeof(float));
                 int first = 0;
                 while (first < v.nplanes) {
                     const size_t need = pv + pp;

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void batch_ratio_kern
     const int local = lid - list[n + e], nrg = g.H / C::G;
     const int rg = local % nrg, pl = local / nrg;
     const SAMPLE* d = static_cast<const SAMPLE*>(S.data[view]) + (long)pl * S.dstride;
-    ratio_tile<SAMPLE, PL>(seg_geom(g, S), XT + (long)(S.plane0 + pl) * xstride, d, rg * C::G, lds, 1.0f);
+    ratio_tile<SAMPLE, PL>(seg_geom(g, S), XT + (long)(S.plane0 + pl) * xstride, d, rg * C::G, lds, S.rscale);
 }
 
 template <typename PL>
@@ -92,7 +92,8 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void batch_col_kernel
     // plane-fastest inside a volume: consecutive blocks reuse the kernel rows
     const int pl = local % S.nplanes, grp = local / S.nplanes;
     const long p = S.plane0 + pl;
-    col_tile<PL>(g, Xin + p * xstride, Xout + p * xstride, hat ? S.Kh[view] : S.K[view], grp, lds);
+    col_tile<PL>(g, Xin + p * xstride, Xout + p * xstride, hat ? S.Kh[view] : S.K[view], grp, lds,
+                 hat && S.khconj ? -1.0f : 1.0f);
 }
 
 // E0 = 10 * nViews of the plane's volume (fmvd_deconvolve_cuda.impl.cu:57-60).

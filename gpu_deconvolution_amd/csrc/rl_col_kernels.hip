@@ -13,8 +13,8 @@
 // multiplied by the DC and Nyquist kernel columns separately and recombined
 // as A*K0 + i*B*KN, whose inverse is again DC + i*Nyquist of the result.
 //
-// Every plan is compiled twice: multiplying by K, and by conj(K) for pass C
-// when the kernel-hat spectrum is the conjugate of the kernel's.
+// Pass C multiplies by conj(K) (ksign = -1) when the kernel-hat spectrum is
+// the conjugate of the kernel's.
 #include "fft_device.hpp"
 #include "hip_check.hpp"
 #include "rl_launch.hpp"
@@ -23,17 +23,18 @@
 
 namespace fmvd {
 
-template <typename PL, bool CONJ>
+template <typename PL>
 __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(ColGeom g, const float2* Xin,
                                                                        float2* Xout, long xstride,
-                                                                       const float2* __restrict__ K, int nplanes)
+                                                                       const float2* __restrict__ K, int nplanes,
+                                                                       float ksign)
 {
     using C = ColCfg<PL>;
     __shared__ float2 lds[C::SQ * C::L];
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     // plane-fastest: consecutive logical blocks reuse the same kernel rows
     const int p = lid % nplanes, grp = lid / nplanes;
-    col_tile<PL, CONJ>(g, Xin + p * xstride, Xout + p * xstride, K, grp, lds);
+    col_tile<PL>(g, Xin + p * xstride, Xout + p * xstride, K, grp, lds, ksign);
 }
 
 #define FMVD_COL_SIZES(X) \
@@ -74,8 +75,7 @@ static void launch_col_plan(const ColGeom& g, const float2* Xin, float2* Xout, l
     using C = ColCfg<PL>;
     if (g.N % C::SQ) throw_msg("row count not divisible by column group");
     dim3 grid((unsigned)(g.N / C::SQ * nplanes));
-    if (conj) launch(col_conv_kernel<PL, true>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes);
-    else launch(col_conv_kernel<PL, false>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes);
+    launch(col_conv_kernel<PL>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes, conj ? -1.0f : 1.0f);
 }
 
 template <int H>

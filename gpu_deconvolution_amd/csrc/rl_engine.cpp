@@ -199,22 +199,22 @@ template <typename S> struct Plan {
     // Cache-resident chunks.  Every plane's RL is independent, so run_batch
     // splits its planes into chunks and runs each chunk's whole iteration
     // chain before the next.  One iteration of a chunk touches its X and ES
-    // (19 MB per plane at FFT 1536^2) and samples of every view, plus every
-    // view's kernel spectra (one when the hat is the conjugate, else two) and
-    // weights; the chunk is the power of two of
-    // planes for which this fits a 200 MiB budget in the 256 MB memory-side
-    // cache (4 planes at FFT 1536^2 with 4 views).  A launch must still fill
-    // the chip: `fill` planes give one residency of the column pass (12
-    // sequences per CU).  When the cache cannot hold that many planes, the
-    // chunk is 4 x fill planes, so that every launch runs several residency
-    // rounds (16 planes at FFT 2560^2).
+    // (19 MB per plane at FFT 1536^2), plus every view's kernel spectra (one
+    // when the hat is the conjugate, else two) and weights; the samples are
+    // streamed with non-temporal loads and stay out of the cache.  The chunk
+    // is the power of two of planes for which this fits a 200 MiB budget in
+    // the 256 MB memory-side cache (8 planes at FFT 1536^2 with 4 views).  A
+    // launch must still fill the chip: `fill` planes give one residency of
+    // the column pass (12 sequences per CU).  When the cache cannot hold that
+    // many planes, the chunk is 4 x fill planes, so that every launch runs
+    // several residency rounds (16 planes at FFT 2560^2).
     int chunk_planes() const
     {
         if (chunkCap > 0) return chunkCap;
         const size_t budget = env_bytes("FMVD_CHUNK_BYTES", (size_t)200 << 20);
         const size_t spectra = khatConj ? 1 : 2;
         const size_t shared = (size_t)nViews * (spectra * xplane() * sizeof(float2) + plane() * sizeof(float));
-        const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float) + (size_t)nViews * plane() * sizeof(S);
+        const size_t per = xplane() * sizeof(float2) + eplane() * sizeof(float);
         const size_t fit = budget > shared ? (budget - shared) / per : 0;
         auto pow2_floor = [](size_t n) {
             size_t p = 1;

@@ -5,6 +5,8 @@
 #include "fft_device.hpp"
 #include "rl_launch.hpp"
 
+#include <type_traits>
+
 namespace fmvd {
 
 __device__ __forceinline__ int pad_index(int i, int n, int half)
@@ -249,7 +251,12 @@ template <typename T, typename PairT = typename PairOf<T>::type>
 __device__ __forceinline__ PairT load_padded_pair(const T* __restrict__ row, int n, int c, int span, int dataW,
                                                   int half, bool pairs)
 {
-    if (pairs && 2 * (c + span) <= dataW) return reinterpret_cast<const PairT*>(row)[n];
+    if (pairs && 2 * (c + span) <= dataW) {
+        // samples are read once per view-iteration: non-temporal, so they do
+        // not displace the spectra and estimates from the memory-side cache
+        if constexpr (std::is_integral_v<PairT>) return __builtin_nontemporal_load(reinterpret_cast<const PairT*>(row) + n);
+        else return reinterpret_cast<const PairT*>(row)[n];
+    }
     if (2 * c >= dataW + half) return pack_pair<T>(row[0], row[0]);
     return pack_pair<T>(row[pad_index(2 * n, dataW, half)], row[pad_index(2 * n + 1, dataW, half)]);
 }
@@ -416,16 +423,18 @@ template <typename PL_> struct ColCfg {
     static_assert(RF == last_radix<PL, +1>(), "inverse exit layout must match forward entry");
 };
 
-// v[i] *= Kr[y] (conj(Kr[y]) for CONJ) at the inverse-entry positions y of
-// thread t; Kr is one kx-row of a kernel spectrum (stride 2 along y).
-template <typename PL, bool CONJ>
-__device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ Kr, int t)
+// v[i] *= Kr[y] at the inverse-entry positions y of thread t, with Kr[y]'s
+// imaginary part times ksign (-1: the conjugate); Kr is one kx-row of a
+// kernel spectrum (stride 2 along y).  The sign is a multiply, not a second
+// code path, so every kernel that inlines this computes the same bits.
+template <typename PL>
+__device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ Kr, int t, float ksign)
 {
     constexpr int RL = last_radix<PL, -1>();
 #pragma unroll
     for (int i = 0; i < PL::P; ++i) {
         const float2 k = Kr[2 * slot_pos<PL, RL>(t, i)];
-        v[i] = CONJ ? cmulc(v[i], k) : cmul(v[i], k);
+        v[i] = cmul(v[i], make_float2(k.x, ksign * k.y));
     }
 }
 
@@ -437,12 +446,12 @@ __device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ K
 // DC + i*Nyquist: its spectrum is split with A[k] = (C[k] + conj C[-k])/2,
 // B[k] = (C[k] - conj C[-k])/(2i), multiplied by the DC and Nyquist kernel
 // columns and recombined as A*K0 + i*B*KN.
-// CONJ: multiply by conj(K) instead.  The spectrum of a real kernel flipped
-// about its centre is the conjugate of the kernel's spectrum, so pass C can
-// read pass A's K.
-template <typename PL, bool CONJ = false>
+// ksign = -1: multiply by conj(K) instead.  The spectrum of a real kernel
+// flipped about its centre is the conjugate of the kernel's spectrum, so pass
+// C can read pass A's K.
+template <typename PL>
 __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, float2* Xout,
-                                         const float2* __restrict__ K, int grp, float2* lds)
+                                         const float2* __restrict__ K, int grp, float2* lds, float ksign)
 {
     using C = ColCfg<PL>;
     constexpr int H = C::H;
@@ -476,16 +485,16 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
                 const float2 d = csub(ck, cm);
                 const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
                 const float2 k0 = K0[2 * k], kn = KN[k];
-                const float2 ak = CONJ ? cmulc(a, k0) : cmul(a, k0);
-                const float2 bk = CONJ ? cmulc(b, kn) : cmul(b, kn);
+                const float2 ak = cmul(a, make_float2(k0.x, ksign * k0.y));
+                const float2 bk = cmul(b, make_float2(kn.x, ksign * kn.y));
                 v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
             }
         } else {
-            col_kmul<PL, CONJ>(v, K + (unsigned)xpos(kx, 0, H), t);
+            col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
         }
         seq_sync<PL>();
     } else {
-        col_kmul<PL, CONJ>(v, K + (unsigned)xpos(kx, 0, H), t);
+        col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
     }
 
     fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);

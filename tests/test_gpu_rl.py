@@ -159,12 +159,14 @@ def test_fewer_planes_than_streams(fm, nplanes):
     assert out.shape == (nplanes, dh, dw)
 
 
-@pytest.mark.parametrize("dh, views, want", [(1024, 4, 4), (2048, 4, 16), (200, 2, 256)])
+@pytest.mark.parametrize("dh, views, want", [(1024, 4, 8), (2048, 4, 16), (200, 2, 256)])
 def test_default_chunk_size(fm, dh, views, want):
-    """Planes per chunk: the power of two whose per-iteration working set fits
-    the 200 MiB cache budget (4 at FFT 1536^2 with 4 views); when the cache
-    cannot hold the planes that fill the chip once, 4 x that count (16 at FFT
-    2560^2); small planes are capped only by 4096 and the batch."""
+    """Planes per chunk: the power of two whose per-iteration working set
+    (spectrum and estimate per plane; kernel spectra and weights per view; the
+    samples are streamed) fits the 200 MiB cache budget (8 at FFT 1536^2 with
+    4 views); when the cache cannot hold the planes that fill the chip once,
+    4 x that count (16 at FFT 2560^2); small planes are capped only by 4096
+    and the batch."""
     k = np.ones((41, 41), np.float32)
     w = [np.ones((dh, dh), np.float32) for _ in range(views)]
     with fm.Deconvolver(dh, dh, w, [k.copy() for _ in range(views)], batch_planes=1024) as dec:
