@@ -106,7 +106,7 @@ int fmvd_set_batch_planes_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
 
 /* Planes whose whole iteration chain runs before the next planes start
  * (0 = automatic: the power of two of planes whose per-iteration working
- * set fits a 200 MiB cache budget, 4 at FFT 1536 x 1536 with 4 views).
+ * set fits a 200 MiB cache budget, 8 at FFT 1536 x 1536 with 4 views).
  * Results do not depend on it.  No counterpart in the reference.  Returns
  * the value in effect. */
 int fmvd_set_chunk_planes_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes);
