@@ -11,6 +11,8 @@
 //   B  row pass      X = R2C(Q > 1 ? D_v / Q : Q), Q = C2R(X)
 //   C  column pass   X = colconv(X, Khat_v / (H W))      X: spectrum of C
 //   D  row pass      C = C2R(X), E += w_v (E C - E), X = XE = R2C(E)/W
+// When Khat_v is K_v flipped about its centre, pass C uses conj(K_v / H) and
+// pass B's R2C carries the remaining 1/W.
 // E (ES) is the spatial estimate on the padded plane, as in the reference; XE
 // is its row spectrum scaled by 1/W, kept so that pass A needs no row FFT.
 // XE is dead from pass A's read to pass D's write, so it shares X with the
