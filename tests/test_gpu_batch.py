@@ -19,6 +19,7 @@ VOLUMES = [
     (3, 2, 250, 240, 31, 3),   # 512 x 512
     (1, 3, 400, 380, 41, 0),   # 512 x 512
     (2, 2, 500, 520, 41, 0),   # 1024 x 1024
+    (2, 2, 151, 171, 31, 0),   # 256 x 256, odd data width
 ]
 
 

@@ -47,6 +47,8 @@ CASES = [
     (1, 2, 50, 2300, 41, 41, 0, 16, 2),       # 128 x 2560 (row length 1280)
     (2, 2, 100, 90, 16, 12, 0, 16, 5),        # 128 x 128, even kernel: explicit kernel-hat spectrum
     (2, 2, 90, 100, 15, 12, 0, 8, 4),         # 128 x 128, odd x even kernel: explicit, 8-bit
+    (2, 3, 101, 77, 15, 15, 0, 16, 5),        # 128 x 128, odd data width (per-sample pad map loads)
+    (2, 2, 77, 131, 13, 17, 1, 8, 4),         # 128 x 256, odd width, 8-bit
 ]
 
 
