@@ -235,10 +235,26 @@ template <typename S> struct Plan {
     {
         ensure_work(np);
         const int c = std::max(1, std::min(np, chunk_planes()));
+        const int lanes = errAcc ? 1 : (int)std::max<size_t>(1, std::min<size_t>(2, env_bytes("FMVD_LANES", 1)));
+        if (lanes > 1 && np > c) {
+            if (!lane2) {
+                lane2.reset(new Stream());
+                laneFork.reset(new Event());
+                laneJoin.reset(new Event());
+            }
+            FMVD_HIP(hipEventRecord(laneFork->e, s));
+            FMVD_HIP(hipStreamWaitEvent(lane2->s, laneFork->e, 0));
+        }
         for (int start = 0, i = 0; start < np; start += c, ++i)
             run_chunk(views, dstride, start, std::min(c, np - start), iterations, init, istride, out, est, ostride,
-                      s, refOffset, i % kTimingStride == 0);
+                      (lanes > 1 && (i & 1)) ? lane2->s : s, refOffset, i % kTimingStride == 0);
+        if (lanes > 1 && np > c) {
+            FMVD_HIP(hipEventRecord(laneJoin->e, lane2->s));
+            FMVD_HIP(hipStreamWaitEvent(s, laneJoin->e, 0));
+        }
     }
+    std::unique_ptr<Stream> lane2;
+    std::unique_ptr<Event> laneFork, laneJoin;
     // With kernel timing on, every kTimingStride-th chunk's launches are timed.
     static constexpr int kTimingStride = 8;
 

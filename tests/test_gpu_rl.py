@@ -139,6 +139,26 @@ def test_batches_and_device_volume_agree(fm):
     assert np.array_equal(from_device(dest, np.float32), ref_est)
 
 
+def test_two_lanes_match_one(fm, monkeypatch):
+    """FMVD_LANES=2 runs alternate chunks on a second stream; chunks touch
+    disjoint planes, so the results equal the single-stream run bit for bit
+    (streaming and device-resident entry points)."""
+    Z, V, dh, dw, k = 9, 3, 70, 96, 13
+    data, kernels, weights = make_case(Z, V, dh, dw, k, k, 16, seed=17)
+    monkeypatch.setenv("FMVD_LANES", "1")
+    ref_out, ref_est, _ = run_gpu(fm, data, kernels, weights, 0, 5, 16, batch=9, chunk=2)
+    monkeypatch.setenv("FMVD_LANES", "2")
+    for b, c in ((9, 2), (9, 1), (5, 2)):
+        out, est, _ = run_gpu(fm, data, kernels, weights, 0, 5, 16, batch=b, chunk=c)
+        assert np.array_equal(out, ref_out) and np.array_equal(est, ref_est), (b, c)
+    dviews = [to_device(np.ascontiguousarray(data[:, v])) for v in range(V)]
+    dout = empty_device((Z, dh, dw), np.uint16)
+    with fm.Deconvolver(dh, dw, list(weights), [x.copy() for x in kernels], batch_planes=9, chunk_planes=2) as dec:
+        dec.run_device([t.ptr for t in dviews], Z, 5, out_ptr=dout.ptr, stream=0)
+        fm.device_synchronize()
+    assert np.array_equal(from_device(dout, np.uint16), ref_out)
+
+
 def test_streaming_order_and_counts(fm):
     # planes with distinct constant levels must come back in z order
     Z, V, dh, dw = 5, 2, 40, 48
