@@ -55,13 +55,8 @@ int orc_pad_index(int i, int n, int half)
 int orc_fft2_c2c_f64(double *data, int h, int w, int inverse)
 {
     plan_t_f64 ph, pw;
-    double *col, *work;
     if (plan_init_f64(&ph, h) || plan_init_f64(&pw, w)) return -1;
-    col = (double *)malloc(sizeof(double) * 2 * (size_t)h);
-    work = (double *)malloc(sizeof(double) * 2 * (size_t)(h > w ? h : w));
-    fft2_run_f64(&ph, &pw, data, h, w, col, work, inverse);
-    free(col);
-    free(work);
+    fft2_run_f64(&ph, &pw, data, h, w, inverse, 1);
     plan_free_f64(&ph);
     plan_free_f64(&pw);
     return 0;
@@ -272,10 +267,14 @@ void orc_transform(const uint16_t *src, int w, int h, int d, int bits,
     const double maxv = (double)((1 << bits) - 1);
     const double scale = (double)(1 << bits);
     double md[12];
-    int i, x, y, z;
+    int i;
+    long row;
     for (i = 0; i < 12; i++) md[i] = (double)m[i];
-    for (z = z0; z < z0 + nz; z++)
-        for (y = 0; y < th; y++)
+    /* target rows are independent: OpenMP over (z, y) */
+#pragma omp parallel for schedule(dynamic, 16)
+    for (row = 0; row < (long)nz * th; row++) {
+        const int z = z0 + (int)(row / th), y = (int)(row % th);
+        int x;
             for (x = 0; x < tw; x++) {
                 double rx = ((md[0] * x + md[1] * y) + md[2] * z) + md[3];
                 double ry = ((md[4] * x + md[5] * y) + md[6] * z) + md[7];
@@ -307,6 +306,7 @@ void orc_transform(const uint16_t *src, int w, int h, int d, int bits,
                 else o = 0;
                 out[((size_t)(z - z0) * th + y) * tw + x] = o;
             }
+    }
 }
 
 /* fmvd_transform.cu:19-67 at z = td/2 (fmvd_transform.impl.cu:155), FP64. */
