@@ -81,6 +81,14 @@ def lib() -> C.CDLL:
         "fmvd_set_chunk_planes_cuda_8": (i, [vp, i]),
         "fmvd_deconvolve_volume_device_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_deconvolve_volume_device_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
+        "fmvd_set_lookahead_cuda_16": (i, [vp, i]),
+        "fmvd_set_lookahead_cuda_8": (i, [vp, i]),
+        "fmvd_set_weights_device_cuda_16": (i, [vp, C.POINTER(vp), vp]),
+        "fmvd_set_weights_device_cuda_8": (i, [vp, C.POINTER(vp), vp]),
+        "fmvd_reconstruct_device_cuda_16": (i, [vp, C.POINTER(vp), i, i, i, C.POINTER(C.POINTER(f)), i, f,
+                                                C.POINTER(vp), i, i, vp, vp, vp]),
+        "fmvd_reconstruct_device_cuda_8": (i, [vp, C.POINTER(vp), i, i, i, C.POINTER(C.POINTER(f)), i, f,
+                                               C.POINTER(vp), i, i, vp, vp, vp]),
         "fmvd_plan_info_cuda_16": (i, [vp, C.POINTER(C.c_int)]),
         "fmvd_plan_info_cuda_8": (i, [vp, C.POINTER(C.c_int)]),
         "fmvd_snap_transform_size": (i, [i]),
@@ -173,7 +181,9 @@ class Deconvolver:
         self.h, self.w = int(data_h), int(data_w)
         self.nviews = len(kernels)
         self.kernels = [np.ascontiguousarray(k, dtype=np.float32) for k in kernels]
-        self.weights = [np.ascontiguousarray(w, dtype=np.float32) for w in weights]
+        self.weights = None if weights is None else [np.ascontiguousarray(w, dtype=np.float32) for w in weights]
+        # callback sequence of the last run: ("get", ok) and ("put", z) in call order
+        self.calls: list = []
         kh, kw = self.kernels[0].shape
         self._dtype = np.uint16 if bits == 16 else np.uint8
         self._ctype = C.c_uint16 if bits == 16 else C.c_uint8
@@ -185,7 +195,9 @@ class Deconvolver:
 
         def source(bufs, offset):
             if self._pos >= len(self._planes_in):
+                self.calls.append(("get", 0))
                 return 0
+            self.calls.append(("get", 1))
             views = self._planes_in[self._pos]
             for v in range(self.nviews):
                 dst = np.ctypeslib.as_array(C.cast(bufs[v], C.POINTER(self._ctype)), shape=(offset + plane,))
@@ -194,6 +206,7 @@ class Deconvolver:
             return 1
 
         def sink(buf):
+            self.calls.append(("put", len(self._planes_out)))
             self._planes_out.append(np.ctypeslib.as_array(buf, shape=(plane,)).copy().reshape(self.h, self.w))
 
         def esink(buf, z, _user):
@@ -215,7 +228,8 @@ class Deconvolver:
             raise ValueError("bits must be 8 or 16")
         self._esink = ESINK_T(esink)
         take_errors()
-        self.plan = init(self.h, self.w, _float_ptr_array(self.weights), _float_ptr_array(self.kernels),
+        wptr = None if self.weights is None else _float_ptr_array(self.weights)
+        self.plan = init(self.h, self.w, wptr, _float_ptr_array(self.kernels),
                          kh, kw, int(psf_type), self.nviews, int(nstreams), self._src, self._snk)
         check()
         if not self.plan:
@@ -232,6 +246,40 @@ class Deconvolver:
 
     def set_batch_planes(self, n: int) -> int:
         return self._set_batch(self.plan, int(n))
+
+    def set_lookahead(self, n: int) -> int:
+        """At most n planes pulled ahead of the sink, in the reference's call
+        order (0 = batched pipeline); returns the value in effect."""
+        L = lib()
+        fn = L.fmvd_set_lookahead_cuda_16 if self.bits == 16 else L.fmvd_set_lookahead_cuda_8
+        return fn(self.plan, int(n))
+
+    def set_weights_device(self, weight_ptrs: Sequence[int], stream: int = 0) -> None:
+        """Weights from device images (dataH x dataW float each), normalised on the device."""
+        L = lib()
+        fn = L.fmvd_set_weights_device_cuda_16 if self.bits == 16 else L.fmvd_set_weights_device_cuda_8
+        arr = (C.c_void_p * self.nviews)(*[C.c_void_p(int(p)) for p in weight_ptrs])
+        rc = fn(self.plan, arr, C.c_void_p(stream or None))
+        check()
+        if rc != 0:
+            raise FmvdError("setting device weights failed")
+
+    def reconstruct_device(self, acquired_ptrs: Sequence[int], src_shape, inverses, border: int, zspacing: float,
+                           registered_ptrs: Sequence[int], nplanes: int, iterations: int, out_ptr: int = 0,
+                           est_ptr: int = 0, stream: int = 0) -> None:
+        """Transform -> border-mask weights -> RL on the device (src_shape = (d, h, w))."""
+        L = lib()
+        fn = L.fmvd_reconstruct_device_cuda_16 if self.bits == 16 else L.fmvd_reconstruct_device_cuda_8
+        d, h, w = src_shape
+        mats = [np.ascontiguousarray(m, dtype=np.float32).reshape(12) for m in inverses]
+        inv = _float_ptr_array(mats)
+        acq = (C.c_void_p * self.nviews)(*[C.c_void_p(int(p)) for p in acquired_ptrs])
+        reg = (C.c_void_p * self.nviews)(*[C.c_void_p(int(p)) for p in registered_ptrs])
+        rc = fn(self.plan, acq, int(w), int(h), int(d), inv, int(border), float(zspacing), reg, int(nplanes),
+                int(iterations), C.c_void_p(out_ptr or None), C.c_void_p(est_ptr or None), C.c_void_p(stream or None))
+        check()
+        if rc != 0:
+            raise FmvdError("device reconstruction failed")
 
     def set_chunk_planes(self, n: int) -> int:
         """Planes per cache-resident chunk (0 = automatic); returns the value in effect."""
@@ -264,7 +312,7 @@ class Deconvolver:
         """planes: sequence over z of per-view (dh, dw) sample arrays.
         Returns (samples[z, dh, dw], estimate[z, dh, dw] or None)."""
         self._planes_in = [[np.ascontiguousarray(v, dtype=self._dtype) for v in p] for p in planes]
-        self._planes_out, self._est_out, self._pos = [], [], 0
+        self._planes_out, self._est_out, self._pos, self.calls = [], [], 0, []
         self._set_esink(self.plan, self._esink if want_estimate else ESINK_T(), None)
         self._run(self.plan, int(iterations))
         check()
@@ -414,6 +462,15 @@ class DeviceArray:
     def download(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         lib().fmvd_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2)
+        check()
+        return out
+
+    def download_planes(self, z0: int, nz: int) -> np.ndarray:
+        """Planes [z0, z0 + nz) along the first axis."""
+        per = int(np.prod(self.shape[1:], dtype=np.int64)) * self.dtype.itemsize
+        assert 0 <= z0 and nz >= 0 and z0 + nz <= self.shape[0]
+        out = np.empty((nz,) + self.shape[1:], self.dtype)
+        lib().fmvd_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + z0 * per), nz * per, 2)
         check()
         return out
 

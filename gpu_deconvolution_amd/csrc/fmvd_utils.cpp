@@ -80,9 +80,14 @@ hipEvent_t pool_get()
     return e;
 }
 
+// A drain error raised inside ~TimedScope, reported by the next timing read.
+std::string g_drain_error;
+
 void drain_locked()
 {
-    for (auto& t : g_pending) {
+    std::vector<TimedLaunch> pending;
+    pending.swap(g_pending);  // a failing pair is dropped, not retried forever
+    for (auto& t : pending) {
         FMVD_HIP(hipEventSynchronize(t.b));
         float ms = 0.f;
         FMVD_HIP(hipEventElapsedTime(&ms, t.a, t.b));
@@ -91,7 +96,15 @@ void drain_locked()
         g_pool.push_back(t.a);
         g_pool.push_back(t.b);
     }
-    g_pending.clear();
+}
+
+// Reports (throws) a drain error kept from a destructor.
+void raise_kept_error_locked()
+{
+    if (g_drain_error.empty()) return;
+    std::string msg;
+    msg.swap(g_drain_error);
+    throw_msg(msg);
 }
 }  // namespace
 
@@ -116,18 +129,22 @@ TimedScope::TimedScope(int kind, hipStream_t s, bool sample) : kind_(kind), s_(s
     t_stop = b_;
 }
 
-TimedScope::~TimedScope() noexcept(false)
+TimedScope::~TimedScope() noexcept
 {
     if (!on_) return;
     std::lock_guard<std::mutex> lk(g_tmu);
-    if (t_start == a_) {  // the scope launched nothing: no sample
-        t_start = t_stop = nullptr;
-        g_pool.push_back(a_);
-        g_pool.push_back(b_);
-        return;
+    try {
+        if (t_start == a_) {  // the scope launched nothing: no sample
+            t_start = t_stop = nullptr;
+            g_pool.push_back(a_);
+            g_pool.push_back(b_);
+            return;
+        }
+        g_pending.push_back({kind_, a_, b_});
+        if (g_pending.size() > 4096) drain_locked();
+    } catch (const std::exception& e) {
+        if (g_drain_error.empty()) g_drain_error = e.what();
     }
-    g_pending.push_back({kind_, a_, b_});
-    if (g_pending.size() > 4096) drain_locked();
 }
 
 }  // namespace fmvd
@@ -203,6 +220,7 @@ void fmvd_kernel_timing_reset(void)
     std::lock_guard<std::mutex> lk(fmvd::g_tmu);
     try {
         fmvd::drain_locked();
+        fmvd::raise_kept_error_locked();
     } catch (const std::exception& e) {
         fmvd::report_error(e.what());
     }
@@ -218,6 +236,7 @@ int fmvd_kernel_timing_read(int kind, double* total_ms, long long* launches)
     std::lock_guard<std::mutex> lk(fmvd::g_tmu);
     FMVD_API_GUARD(-1, {
         fmvd::drain_locked();
+        fmvd::raise_kept_error_locked();
         if (total_ms) *total_ms = fmvd::g_total_ms[kind];
         if (launches) *launches = fmvd::g_count[kind];
         return 0;

@@ -177,9 +177,29 @@ template <typename S> int get_next_plane(Interactive* ia, S** data, int offset)
         DeleteLocalRef(env, jarr);
         return 0;
     }
+    // the upcall's array must hold nViews planes of planesize samples; a
+    // short one ends the stream with a Java exception instead of reading
+    // past it (the reference copied without checking)
+    if (GetArrayLength(env, jarr) < ia->nViews) {
+        DeleteLocalRef(env, jarr);
+        ThrowException(env, "getNextPlane returned fewer planes than views");
+        return 0;
+    }
     for (int v = 0; v < ia->nViews; ++v) {
         jarray view = (jarray)GetObjectArrayElement(env, jarr, v);
+        if (view == nullptr || GetArrayLength(env, view) < ia->planesize) {
+            DeleteLocalRef(env, view);
+            DeleteLocalRef(env, jarr);
+            ThrowException(env, "getNextPlane returned a view with fewer than planesize samples");
+            return 0;
+        }
         void* src = GetPrimitiveArrayCritical(env, view);
+        if (!src) {
+            DeleteLocalRef(env, view);
+            DeleteLocalRef(env, jarr);
+            ThrowException(env, "not enough memory");
+            return 0;
+        }
         memcpy(data[v] + offset, src, (size_t)ia->planesize * sizeof(S));
         ReleasePrimitiveArrayCritical(env, view, src, JNI_ABORT);
         DeleteLocalRef(env, view);
@@ -191,9 +211,13 @@ template <typename S> int get_next_plane(Interactive* ia, S** data, int offset)
 int get_next_plane_8(unsigned char** d, int o) { return get_next_plane<unsigned char>(ia_8, d, o); }
 int get_next_plane_16(unsigned short** d, int o) { return get_next_plane<unsigned short>(ia_16, d, o); }
 
+// With a Java exception pending (a failed getNextPlane) no further upcall
+// may be made; the remaining planes are dropped and the exception reaches
+// the caller when the native method returns.
 void return_next_plane_8(unsigned char* data)
 {
     JNIEnv* env = AttachCurrentThread(ia_8->jvm);
+    if (ExceptionCheck(env)) return;
     jbyteArray a = NewByteArray(env, ia_8->planesize);
     SetByteArrayRegion(env, a, 0, ia_8->planesize, (const jbyte*)data);
     CallStaticVoidMethod1(env, ia_8->callingClass, ia_8->returnNextPlane, a);
@@ -203,6 +227,7 @@ void return_next_plane_8(unsigned char* data)
 void return_next_plane_16(unsigned short* data)
 {
     JNIEnv* env = AttachCurrentThread(ia_16->jvm);
+    if (ExceptionCheck(env)) return;
     jshortArray a = NewShortArray(env, ia_16->planesize);
     SetShortArrayRegion(env, a, 0, ia_16->planesize, (const jshort*)data);
     CallStaticVoidMethod1(env, ia_16->callingClass, ia_16->returnNextPlane, a);
@@ -464,6 +489,14 @@ JNIEXPORT void JNICALL Java_fastspim_NativeSPIMReconstructionCuda_deconvolve_1in
         delete ia;
         return;
     }
+    // The Java side may be a bounded producer: keep the reference's call
+    // order, at most nStreams planes pulled ahead of returnNextPlane
+    // (fastspim_NativeSPIMReconstructionCuda.cpp:482, fmvd_deconvolve.impl.cpp:216-245).
+    // FMVD_JNI_LOOKAHEAD overrides the bound; 0 selects the batched pipeline.
+    int lookahead = nStreams;
+    if (const char* e = getenv("FMVD_JNI_LOOKAHEAD"); e && *e) lookahead = atoi(e);
+    if (bitDepth == 8) fmvd_set_lookahead_cuda_8((fmvd_plan_cuda_8*)ia->plan, lookahead);
+    else fmvd_set_lookahead_cuda_16((fmvd_plan_cuda_16*)ia->plan, lookahead);
     ia->callingClass = (jclass)NewGlobalRef(env, clazz);
     if (bitDepth == 8) ia_8 = ia;
     else ia_16 = ia;

@@ -36,6 +36,7 @@
 #include "host_psf.hpp"
 #include "rl_launch.hpp"
 #include "timing.hpp"
+#include "transform.hpp"
 
 namespace fmvd {
 
@@ -65,6 +66,9 @@ template <typename S> struct Plan {
     fmvd_estimate_sink_t esink = nullptr;
     void* euser = nullptr;
     int batchCap = 0, chunkCap = 0;
+    // > 0: deconvolve_planes keeps at most this many planes pulled from the
+    // datasource ahead of the datasink, with the reference's call order
+    int lookahead = 0;
     int cus = 256;  // compute units of the plan's device
     Variants var;
 
@@ -134,7 +138,7 @@ template <typename S> struct Plan {
         if (dH < 1 || dW < 1 || kH < 1 || kW < 1 || nV < 1)
             throw_msg("invalid plan dimensions (dataH, dataW, kernelH, kernelW, nViews must be >= 1)");
         if (psf < 0 || psf > 3) throw_msg("invalid PSF type (expected 0..3)");
-        if (!h_Weights || !h_Kernel) throw_msg("weights and kernels must not be NULL");
+        if (!h_Kernel) throw_msg("kernels must not be NULL");
         dataH = dH;
         dataW = dW;
         kernelH = kH;
@@ -184,7 +188,16 @@ template <typename S> struct Plan {
                 upload(Kh[v], kernel_spectrum_transposed(khat.data() + (size_t)v * len, kernelW, kernelH,
                                                          fftH, fftW, 1.0 / ((double)fftH * fftW)));
         }
-        std::vector<float> wn = normalized_weights(h_Weights, dataW, dataH, nViews);
+        // NULL weights (an extension): every view weighs 1 until
+        // set_weights_device replaces them
+        std::vector<float> ones;
+        std::vector<const float*> wp(nViews);
+        if (!h_Weights) ones.assign(plane(), 1.0f);
+        for (int v = 0; v < nViews; ++v) {
+            wp[v] = h_Weights ? h_Weights[v] : ones.data();
+            if (!wp[v]) throw_msg("weight image must not be NULL");
+        }
+        std::vector<float> wn = normalized_weights(wp.data(), dataW, dataH, nViews);
         for (int v = 0; v < nViews; ++v) {
             std::vector<float> one(wn.begin() + (size_t)v * plane(), wn.begin() + (size_t)(v + 1) * plane());
             upload(Wn[v], one);
@@ -375,6 +388,47 @@ template <typename S> struct Plan {
         FMVD_HIP(hipStreamSynchronize(s));
     }
 
+    // Replaces the plan's weights with nViews device weight images (dataH x
+    // dataW floats each), normalised over views on the device in the host
+    // rule's float operations.  Enqueued on s: later runs on s see them.
+    DeviceBuffer wsum;
+    void set_weights_device(const float* const* dw, hipStream_t s)
+    {
+        if (!dw) throw_msg("d_weights must not be NULL");
+        for (int v = 0; v < nViews; ++v)
+            if (!dw[v]) throw_msg("weight image must not be NULL");
+        wsum.alloc(sizeof(float) * plane());
+        const long n = (long)plane();
+        for (int v = 0; v < nViews; ++v) launch_weight_accumulate(wsum.as<float>(), dw[v], n, v == 0, s);
+        for (int v = 0; v < nViews; ++v) launch_weight_normalize(Wn[v].as<float>(), dw[v], wsum.as<float>(), n, s);
+    }
+
+    // GPU-resident chain: view v's acquired volume (w x h x d, device) is
+    // transformed with inv[v] into registered[v] (nplanes planes of this
+    // plan's dataH x dataW); the border mask of every view becomes the
+    // plan's weights; then RL as run_volume.  Nothing returns to the host.
+    DeviceBuffer masks;
+    void reconstruct_device(const S* const* acquired, int w, int h, int d, const float* const* inv, int border,
+                            float zspacing, S* const* registered, int nplanes, int iterations, S* out, float* est,
+                            hipStream_t s)
+    {
+        if (!acquired || !inv || !registered) throw_msg("reconstruct: NULL argument");
+        if (nplanes < 1) throw_msg("reconstruct: nPlanes must be >= 1");
+        masks.alloc(sizeof(float) * plane() * nViews);
+        std::vector<const float*> mp(nViews);
+        std::vector<const S*> rp(nViews);
+        for (int v = 0; v < nViews; ++v) {
+            if (!acquired[v] || !inv[v] || !registered[v]) throw_msg("reconstruct: NULL view");
+            transform_volume<S>(acquired[v], w, h, d, registered[v], dataW, dataH, nplanes, inv[v], s);
+            float* m = masks.as<float>() + (size_t)v * plane();
+            transform_mask(w, h, d, dataW, dataH, nplanes, inv[v], border, zspacing, m, s);
+            mp[v] = m;
+            rp[v] = registered[v];
+        }
+        set_weights_device(mp.data(), s);
+        run_volume(rp.data(), nplanes, iterations, nullptr, out, est, s);
+    }
+
     void run_volume(const S* const* dviews, int nplanes, int iterations, const float* init, S* out,
                     float* est, hipStream_t s)
     {
@@ -428,6 +482,7 @@ template <typename S> struct Plan {
         if (!source || !sink) throw_msg("datasource and datasink must be set");
         if (iterations < 0) throw_msg("negative iteration count");
         FMVD_HIP(hipSetDevice(device));
+        if (lookahead > 0) return deconvolve_planes_bounded(iterations);
         const int nb = batch_planes();
         const size_t pl = plane();
         const size_t inBytes = sizeof(S) * pl * nViews * (size_t)nb;
@@ -506,6 +561,64 @@ template <typename S> struct Plan {
             slot = ns;
         }
     }
+
+    // fmvd_deconvolve.impl.cpp:205-300 call order with at most L = lookahead
+    // planes pulled ahead of the sink: L source calls, then per plane a sink
+    // call followed by a source call (until one returns 0), then the sinks
+    // of the remaining planes.  Planes run in batches of L: the batch's
+    // results are sunk one by one, and each sink makes room for the next
+    // plane's source call.  A source that runs dry during the first L calls
+    // ends the stream there (the reference kept calling it).
+    void deconvolve_planes_bounded(int iterations)
+    {
+        const int nb = lookahead;
+        const size_t pl = plane();
+        DeviceBuffer din, dout, dest;
+        PinnedBuffer hin, hout, hest;
+        din.alloc(sizeof(S) * pl * nViews * (size_t)nb);
+        hin.alloc(sizeof(S) * pl * nViews * (size_t)nb);
+        dout.alloc(sizeof(S) * pl * nb);
+        hout.alloc(sizeof(S) * pl * nb);
+        if (esink) {
+            dest.alloc(sizeof(float) * pl * nb);
+            hest.alloc(sizeof(float) * pl * nb);
+        }
+        ensure_work(nb);
+        std::vector<S*> hviews(nViews);
+        std::vector<const S*> dviews(nViews);
+        for (int v = 0; v < nViews; ++v) {
+            hviews[v] = hin.as<S>() + (size_t)v * nb * pl;
+            dviews[v] = din.as<S>() + (size_t)v * nb * pl;
+        }
+        bool exhausted = false;
+        int n = 0;
+        while (n < nb && source(hviews.data(), (int)(n * pl))) ++n;
+        exhausted = n < nb;
+        hipStream_t cs = compute->s;
+        int zout = 0;
+        while (n > 0) {
+            for (int v = 0; v < nViews; ++v)
+                FMVD_HIP(hipMemcpyAsync(const_cast<S*>(dviews[v]), hviews[v], sizeof(S) * pl * n,
+                                        hipMemcpyHostToDevice, cs));
+            run_batch(dviews.data(), (long)pl, n, iterations, nullptr, 0, dout.as<S>(),
+                      esink ? dest.as<float>() : nullptr, (long)pl, cs);
+            FMVD_HIP(hipMemcpyAsync(hout.p, dout.p, sizeof(S) * pl * n, hipMemcpyDeviceToHost, cs));
+            if (esink)
+                FMVD_HIP(hipMemcpyAsync(hest.p, dest.p, sizeof(float) * pl * n, hipMemcpyDeviceToHost, cs));
+            FMVD_HIP(hipStreamSynchronize(cs));
+            int m = 0;
+            for (int j = 0; j < n; ++j) {
+                if (esink) esink(hest.as<float>() + (size_t)j * pl, zout, euser);
+                sink(hout.as<S>() + (size_t)j * pl);
+                ++zout;
+                if (!exhausted) {
+                    if (source(hviews.data(), (int)(m * pl))) ++m;
+                    else exhausted = true;
+                }
+            }
+            n = m;
+        }
+    }
 };
 
 }  // namespace fmvd
@@ -566,8 +679,9 @@ template <typename PlanT> static int set_variants(PlanT* plan, const int* v)
 {
     if (!plan || !v) return -1;
     const int nc = fmvd::col_variants(plan->impl.fftH), nr = fmvd::row_variants(plan->impl.N);
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < 4; ++k)
         if (v[k] < 0 || v[k] >= (k == 0 ? nc : nr)) return -1;
+    if (v[4] != 0) return -1;  // the egress kernel has no FFT plan
     fmvd::Variants& x = plan->impl.var;
     x.column = v[0];
     x.ratio = v[1];
@@ -673,6 +787,64 @@ int fmvd_set_chunk_planes_cuda_8(struct fmvd_plan_cuda_8* plan, int planes)
     if (!plan) return -1;
     plan->impl.chunkCap = planes > 0 ? planes : 0;
     return plan->impl.chunk_planes();
+}
+
+int fmvd_set_lookahead_cuda_16(struct fmvd_plan_cuda_16* plan, int planes)
+{
+    if (!plan) return -1;
+    plan->impl.lookahead = planes > 0 ? planes : 0;
+    return plan->impl.lookahead;
+}
+
+int fmvd_set_lookahead_cuda_8(struct fmvd_plan_cuda_8* plan, int planes)
+{
+    if (!plan) return -1;
+    plan->impl.lookahead = planes > 0 ? planes : 0;
+    return plan->impl.lookahead;
+}
+
+int fmvd_set_weights_device_cuda_16(struct fmvd_plan_cuda_16* plan, const float* const* d_weights, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        plan->impl.set_weights_device(d_weights, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_set_weights_device_cuda_8(struct fmvd_plan_cuda_8* plan, const float* const* d_weights, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        plan->impl.set_weights_device(d_weights, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_reconstruct_device_cuda_16(struct fmvd_plan_cuda_16* plan, const unsigned short* const* d_acquired, int w,
+                                    int h, int d, const float* const* h_inverse, int border, float zspacing,
+                                    unsigned short* const* d_registered, int nPlanes, int iterations,
+                                    unsigned short* d_out, float* d_estimate, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        plan->impl.reconstruct_device(d_acquired, w, h, d, h_inverse, border, zspacing, d_registered, nPlanes,
+                                      iterations, d_out, d_estimate, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_reconstruct_device_cuda_8(struct fmvd_plan_cuda_8* plan, const unsigned char* const* d_acquired, int w,
+                                   int h, int d, const float* const* h_inverse, int border, float zspacing,
+                                   unsigned char* const* d_registered, int nPlanes, int iterations,
+                                   unsigned char* d_out, float* d_estimate, void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        plan->impl.reconstruct_device(d_acquired, w, h, d, h_inverse, border, zspacing, d_registered, nPlanes,
+                                      iterations, d_out, d_estimate, (hipStream_t)stream);
+        return 0;
+    })
 }
 
 int fmvd_deconvolve_volume_device_cuda_16(const struct fmvd_plan_cuda_16* plan,

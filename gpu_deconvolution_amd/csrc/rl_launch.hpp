@@ -74,6 +74,12 @@ std::vector<float2> col_pass_twiddles(int H, int variant);
 void launch_error_sums(const RowGeom& g, const float* ES, long estride, const float* ref, long rstride, int nplanes,
                        double* acc, hipStream_t s);
 
+// Per-pixel view normalisation of n-pixel device weight images, in the host
+// rule's float operations: sum = ((0 + w_0) + w_1) + ... accumulated one
+// view per launch (first = view 0), then out_v = sum > 0 ? w_v / sum : w_v.
+void launch_weight_accumulate(float* sum, const float* w, long n, bool first, hipStream_t s);
+void launch_weight_normalize(float* out, const float* w, const float* sum, long n, hipStream_t s);
+
 bool row_size_supported(int N);
 bool col_size_supported(int H);
 // Number of compiled FFT-plan variants for a row / column length (>= 1 when

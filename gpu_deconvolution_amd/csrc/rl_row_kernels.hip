@@ -151,6 +151,42 @@ void launch_error_sums(const RowGeom& g, const float* ES, long estride, const fl
     FMVD_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------- device weights
+// fmvd_deconvolve.impl.cpp:86-124 / normalizeWeights_kernel
+// (fmvd_deconvolve_cuda.cu:160-199) on unpadded weight images: the sum over
+// views is accumulated in view order, then each view is divided by it where
+// it is positive.  The same float operations as the host rule, so weights
+// normalised here equal the host's bit for bit.
+__global__ void weight_accumulate_kernel(float* __restrict__ sum, const float* __restrict__ w, long n, int first)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sum[i] = (first ? 0.0f : sum[i]) + w[i];
+}
+
+__global__ void weight_normalize_kernel(float* __restrict__ out, const float* __restrict__ w,
+                                        const float* __restrict__ sum, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float s = sum[i], v = w[i];
+    out[i] = s > 0.0f ? v / s : v;
+}
+
+void launch_weight_accumulate(float* sum, const float* w, long n, bool first, hipStream_t s)
+{
+    if (n <= 0) return;
+    weight_accumulate_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(sum, w, n, first ? 1 : 0);
+    FMVD_LAUNCH_CHECK();
+}
+
+void launch_weight_normalize(float* out, const float* w, const float* sum, long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    weight_normalize_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(out, w, sum, n);
+    FMVD_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------- dispatchers
 #define FMVD_ROW_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(768) X(1024) X(1280) X(1536) X(1792) X(2048)
 

@@ -7,6 +7,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace fmvd {
 
 enum TimingKind {
@@ -27,7 +29,9 @@ bool timing_on();
 class TimedScope {
 public:
     TimedScope(int kind, hipStream_t s, bool sample = true);
-    ~TimedScope() noexcept(false);
+    // Never throws (it may run while an exception unwinds): a failure to
+    // drain the recorded events is kept and reported by the next timing read.
+    ~TimedScope() noexcept;
     TimedScope(const TimedScope&) = delete;
     TimedScope& operator=(const TimedScope&) = delete;
 
@@ -37,6 +41,8 @@ private:
     bool on_;
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
+
+static_assert(std::is_nothrow_destructible_v<TimedScope>);
 
 // The event pair of the innermost open TimedScope on this thread, taken by
 // the first launch inside it (both null when none is pending).

@@ -22,6 +22,7 @@
 #include "../../include/fmvd_transform.h"
 #include "hip_check.hpp"
 #include "timing.hpp"
+#include "transform.hpp"
 
 namespace fmvd {
 
@@ -342,6 +343,30 @@ static void transform_files(S** h_data, int w, int h, int d, int tw, int th, int
     cleanup();
 }
 
+void transform_volume_u16(const uint16_t* src, int w, int h, int d, uint16_t* dst, int tw, int th, int td,
+                          const float* inv, hipStream_t s)
+{
+    check_dims(w, h, d, tw, th, td);
+    if (!src || !dst || !inv) throw_msg("transform: NULL argument");
+    launch_transform<uint16_t>(src, w, h, d, dst, tw, th, 0, td, to_mat(inv), s);
+}
+
+void transform_volume_u8(const uint8_t* src, int w, int h, int d, uint8_t* dst, int tw, int th, int td,
+                         const float* inv, hipStream_t s)
+{
+    check_dims(w, h, d, tw, th, td);
+    if (!src || !dst || !inv) throw_msg("transform: NULL argument");
+    launch_transform<uint8_t>(src, w, h, d, dst, tw, th, 0, td, to_mat(inv), s);
+}
+
+void transform_mask(int w, int h, int d, int tw, int th, int td, const float* inv, int border, float zspacing,
+                    float* out, hipStream_t s)
+{
+    check_dims(w, h, d, tw, th, td);
+    if (!out || !inv) throw_msg("mask: NULL argument");
+    launch_mask(out, w, h, d, tw, th, td, to_mat(inv), border, zspacing, s);
+}
+
 }  // namespace fmvd
 
 extern "C" {
@@ -370,10 +395,7 @@ int fmvd_transform_device_8(const unsigned char* d_src, int w, int h, int d, uns
                             int tw, int th, int td, const float* h_inverse, void* stream)
 {
     FMVD_API_GUARD(-1, {
-        fmvd::check_dims(w, h, d, tw, th, td);
-        if (!d_src || !d_dst || !h_inverse) fmvd::throw_msg("transform: NULL argument");
-        fmvd::launch_transform<uint8_t>(d_src, w, h, d, d_dst, tw, th, 0, td, fmvd::to_mat(h_inverse),
-                                        (hipStream_t)stream);
+        fmvd::transform_volume_u8(d_src, w, h, d, d_dst, tw, th, td, h_inverse, (hipStream_t)stream);
         return 0;
     })
 }
@@ -382,10 +404,7 @@ int fmvd_transform_device_16(const unsigned short* d_src, int w, int h, int d, u
                              int tw, int th, int td, const float* h_inverse, void* stream)
 {
     FMVD_API_GUARD(-1, {
-        fmvd::check_dims(w, h, d, tw, th, td);
-        if (!d_src || !d_dst || !h_inverse) fmvd::throw_msg("transform: NULL argument");
-        fmvd::launch_transform<uint16_t>(d_src, w, h, d, d_dst, tw, th, 0, td, fmvd::to_mat(h_inverse),
-                                         (hipStream_t)stream);
+        fmvd::transform_volume_u16(d_src, w, h, d, d_dst, tw, th, td, h_inverse, (hipStream_t)stream);
         return 0;
     })
 }
@@ -394,10 +413,7 @@ int fmvd_transform_mask_device(int w, int h, int d, int tw, int th, int td, cons
                                int border, float zspacing, float* d_mask, void* stream)
 {
     FMVD_API_GUARD(-1, {
-        fmvd::check_dims(w, h, d, tw, th, td);
-        if (!d_mask || !h_inverse) fmvd::throw_msg("mask: NULL argument");
-        fmvd::launch_mask(d_mask, w, h, d, tw, th, td, fmvd::to_mat(h_inverse), border, zspacing,
-                          (hipStream_t)stream);
+        fmvd::transform_mask(w, h, d, tw, th, td, h_inverse, border, zspacing, d_mask, (hipStream_t)stream);
         return 0;
     })
 }

@@ -112,6 +112,50 @@ int fmvd_set_batch_planes_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
 int fmvd_set_chunk_planes_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes);
 int fmvd_set_chunk_planes_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
 
+/* Bound on the planes pulled from the datasource ahead of the datasink
+ * (0 = the batched pipeline above, the default).  With planes = L > 0,
+ * fmvd_deconvolve_planes_cuda_* makes the reference's callback sequence
+ * (fmvd_deconvolve.impl.cpp:216-245,288-294): L source calls, then one
+ * sink call followed by one source call per plane until a source call
+ * returns 0, then the sinks of the planes still held; with L = nstreams this
+ * is exactly the reference's order.  Planes then run in batches of L with no
+ * overlap of host callbacks and device work.  The JNI interactive session
+ * uses L = nStreams.  No counterpart in the reference.  Returns the value in
+ * effect. */
+int fmvd_set_lookahead_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes);
+int fmvd_set_lookahead_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes);
+
+/* Replaces the plan's view weights with nViews device weight images
+ * (d_weights: host array of nViews device pointers to dataH*dataW floats),
+ * normalised over views on the device with the float operations of
+ * prepareWeights (fmvd_deconvolve.impl.cpp:86-124): bit-identical to
+ * passing the same images as h_Weights at init.  Enqueued on `stream`; runs
+ * enqueued later on the same stream use them.  fmvd_initialize_cuda_* also
+ * accepts h_Weights = NULL (every view weighs 1) for plans whose weights come
+ * from here.  No counterpart in the reference.  Returns 0 on success. */
+int fmvd_set_weights_device_cuda_16(struct fmvd_plan_cuda_16 *plan, const float *const *d_weights, void *stream);
+int fmvd_set_weights_device_cuda_8(struct fmvd_plan_cuda_8 *plan, const float *const *d_weights, void *stream);
+
+/* GPU-resident transform -> weights -> RL chain (no host round trip): view
+ * v's acquired volume d_acquired[v] (w x h x d samples, device) is
+ * resampled with h_inverse[v] (12 floats, as transform_cuda_*) into
+ * d_registered[v] (nPlanes planes of dataW x dataH, device, caller-owned);
+ * each view's border mask (fmvd_transform.cu:19-67, plane z = nPlanes/2,
+ * `border`, `zspacing`) becomes its weight image, normalised as
+ * fmvd_set_weights_device_cuda_*; then RL as
+ * fmvd_deconvolve_volume_device_cuda_* into d_out / d_estimate.  The
+ * reference chains these through files (transform_cuda_* writes the volume and
+ * the mask, the deconvolution reads them back).  Enqueued on `stream`;
+ * returns 0 on success. */
+int fmvd_reconstruct_device_cuda_16(struct fmvd_plan_cuda_16 *plan, const unsigned short *const *d_acquired,
+                                    int w, int h, int d, const float *const *h_inverse, int border,
+                                    float zspacing, unsigned short *const *d_registered, int nPlanes,
+                                    int iterations, unsigned short *d_out, float *d_estimate, void *stream);
+int fmvd_reconstruct_device_cuda_8(struct fmvd_plan_cuda_8 *plan, const unsigned char *const *d_acquired,
+                                   int w, int h, int d, const float *const *h_inverse, int border,
+                                   float zspacing, unsigned char *const *d_registered, int nPlanes,
+                                   int iterations, unsigned char *d_out, float *d_estimate, void *stream);
+
 /* GPU-resident volume deconvolution: d_views[v] (host array of nViews device
  * pointers) each point to nPlanes planes of dataW*dataH samples in device
  * memory.  Results go to d_out (nPlanes planes of samples) and/or d_estimate
@@ -138,8 +182,9 @@ int fmvd_deconvolve_volume_device_cuda_8(const struct fmvd_plan_cuda_8 *plan,
  * safe to call while the plan is running. */
 int fmvd_autotune_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes, int *chosen, float *times_ms);
 int fmvd_autotune_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes, int *chosen, float *times_ms);
-/* Sets the variants directly (variants[5] as above).  Returns -1 if a value is
- * out of range for this plan's extents. */
+/* Sets the variants directly (variants[5] as above; the egress kernel has no
+ * FFT plan, so variants[4] must be 0).  Returns -1 if a value is out of range
+ * for this plan's extents. */
 int fmvd_set_variants_cuda_16(struct fmvd_plan_cuda_16 *plan, const int *variants);
 int fmvd_set_variants_cuda_8(struct fmvd_plan_cuda_8 *plan, const int *variants);
 

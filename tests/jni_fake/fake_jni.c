@@ -51,6 +51,17 @@ static unsigned char **g_ret = NULL;
 
 static int MID_GET = 1, MID_RET = 2;
 
+/* upcall log: 'G' a served plane, 'g' end of data (null), 'R' a returned plane */
+static char g_log[8192];
+static int g_loglen = 0;
+static void log_call(char c)
+{
+    if (g_loglen < (int)sizeof g_log - 1) g_log[g_loglen++] = c;
+    g_log[g_loglen] = 0;
+}
+/* a served plane whose view `g_short_view` has only `g_short_len` samples */
+static int g_short_plane = -1, g_short_view = 0, g_short_len = 0;
+
 static Obj *new_obj(int kind, int len, size_t bytes)
 {
     Obj *o = (Obj *)calloc(1, sizeof(Obj));
@@ -102,13 +113,19 @@ static void *GetStaticMethodID(void *env, void *cls, const char *name, const cha
 static void *CallStaticObjectMethod(void *env, void *cls, void *mid, ...)
 {
     (void)env; (void)cls;
-    if (mid != &MID_GET || g_next >= g_nplanes) return NULL;
+    if (mid != &MID_GET) return NULL;
+    if (g_next >= g_nplanes) {
+        log_call('g');
+        return NULL;
+    }
+    log_call('G');
     const int ak = g_kind == 16 ? K_SHORTARR : K_BYTEARR;
     const size_t es = g_kind == 16 ? 2 : 1;
     Obj *arr = new_obj(K_OBJARR, g_nviews, sizeof(Obj *) * g_nviews);
     for (int v = 0; v < g_nviews; ++v) {
-        Obj *p = new_obj(ak, g_planesize, es * g_planesize);
-        memcpy(p->data, g_planes + ((size_t)g_next * g_nviews + v) * g_planesize * es, es * g_planesize);
+        const int len = (g_next == g_short_plane && v == g_short_view) ? g_short_len : g_planesize;
+        Obj *p = new_obj(ak, len, es * len);
+        memcpy(p->data, g_planes + ((size_t)g_next * g_nviews + v) * g_planesize * es, es * len);
         ((Obj **)arr->data)[v] = p;
     }
     g_next++;
@@ -122,6 +139,8 @@ static void CallStaticVoidMethod(void *env, void *cls, void *mid, ...)
     Obj *o = va_arg(ap, Obj *);
     va_end(ap);
     if (mid != &MID_RET || !o) return;
+    if (g_pending) g_unexpected++; /* JNI forbids calls with an exception pending */
+    log_call('R');
     if (g_nret == g_ret_cap) {
         g_ret_cap = g_ret_cap ? 2 * g_ret_cap : 16;
         g_ret = (unsigned char **)realloc(g_ret, sizeof(unsigned char *) * g_ret_cap);
@@ -269,8 +288,19 @@ int fj_planes_served(void) { return g_next; }
 int fj_returned_count(void) { return g_nret; }
 const void *fj_returned_plane(int i) { return (i >= 0 && i < g_nret) ? g_ret[i] : NULL; }
 
+const char *fj_call_log(void) { return g_log; }
+void fj_set_short_view(int plane, int view, int len)
+{
+    g_short_plane = plane;
+    g_short_view = view;
+    g_short_len = len;
+}
+
 void fj_reset(void)
 {
+    g_loglen = 0;
+    g_log[0] = 0;
+    g_short_plane = -1;
     while (g_objs) {
         Obj *n = g_objs->next;
         free(g_objs->data);

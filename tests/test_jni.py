@@ -42,6 +42,7 @@ FAKE_SIGS = {
     "fj_set_methods_present": (None, [I]), "fj_set_planes": (None, [I, I, I, I, P]),
     "fj_planes_served": (I, []), "fj_returned_count": (I, []), "fj_returned_plane": (P, [I]),
     "fj_reset": (None, []), "fj_string_value": (C.c_char_p, [P]),
+    "fj_call_log": (C.c_char_p, []), "fj_set_short_view": (None, [I, I, I]),
 }
 
 
@@ -232,8 +233,13 @@ def test_interactive_session_matches_c_api(jni, fm, tmp_path, bits, psf):
         planes = np.ascontiguousarray(data.astype(dt))
         jni.f.fj_set_planes(bits, Z, V, dh * dw, planes.ctypes.data)
         before = jni.f.fj_returned_count()
+        log0 = len(jni.f.fj_call_log())
         jni.call(f"deconvolve_1interactive{bits}", iters)
         assert jni.exception() is None
+        # the reference's upcall order with nStreams = min(dataD, 3) = 3
+        # (fmvd_deconvolve.impl.cpp:216-245,288-294): 3 gets, (return, get)
+        # pairs until a get yields null, then the remaining returns
+        assert jni.f.fj_call_log().decode()[log0:] == "GGGRGRGRgRR"
         assert jni.f.fj_planes_served() == Z
         assert jni.f.fj_returned_count() - before == Z
         got = np.stack([np.ctypeslib.as_array(C.cast(jni.f.fj_returned_plane(before + z), C.POINTER(
@@ -293,3 +299,23 @@ def test_transform_matches_c_api(jni, fm, tmp_path, bits):
     assert jni.exception() is None
     assert open(out_j, "rb").read() == open(out_c, "rb").read()
     assert open(mask_j, "rb").read() == open(mask_c, "rb").read()
+
+
+@pytest.mark.gpu
+def test_interactive_short_view_raises(jni, fm, tmp_path):
+    """A getNextPlane array whose view is shorter than dataW*dataH ends the
+    stream with a RuntimeException; no upcall follows the exception."""
+    from gpu_util import make_case
+    Z, V, dh, dw, k = 5, 2, 40, 50, 9
+    data, kernels, weights = make_case(Z, V, dh, dw, k, k, 16, seed=3)
+    _, kf, wf = _write_case(tmp_path, data, kernels, weights, 16)
+    jni.call("deconvolve_1init", dw, dh, Z, jni.strings(wf), jni.strings(kf), k, k, 0, V, 16)
+    assert jni.exception() is None
+    planes = np.ascontiguousarray(data)
+    jni.f.fj_set_planes(16, Z, V, dh * dw, planes.ctypes.data)
+    jni.f.fj_set_short_view(3, 1, dh * dw - 5)
+    jni.call("deconvolve_1interactive16", 2)
+    assert jni.exception() == "getNextPlane returned a view with fewer than planesize samples"
+    assert jni.f.fj_call_log().decode() == "GGGRG"
+    jni.call("deconvolve_1quit16")
+    assert jni.exception() is None
