@@ -211,6 +211,45 @@ template <int DIR> struct Dft<7, DIR> {
     }
 };
 
+// Odd prime R (11, 13) from the definition, folded over conjugate pairs as
+// in the radix-5 and radix-7 DFTs: with b_j = v_j + v_{R-j} and
+// d_j = v_j - v_{R-j} (j = 1 .. (R-1)/2),
+//   X_k = v_0 + sum_j cos(2 pi j k / R) b_j + i DIR sum_j sin(2 pi j k / R) d_j,
+//   X_{R-k} = the same with the sine sum negated.
+// The constants come from the compile-time trigonometry above.
+template <int R, int DIR> struct DftOddPrime {
+    static FMVD_HD void run(float2* v)
+    {
+        constexpr int H = (R - 1) / 2;
+        float2 b[H + 1], d[H + 1];
+        const float2 a = v[0];
+        float2 sum = a;
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            b[j] = cadd(v[j], v[R - j]);
+            d[j] = csub(v[j], v[R - j]);
+            sum = cadd(sum, b[j]);
+        }
+        v[0] = sum;
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            float2 m = a, e = make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int j = 1; j <= H; ++j) {
+                const float c = (float)ct_cos2pi((long)j * k % R, R);
+                const float sn = (float)(DIR * ct_sin2pi((long)j * k % R, R));
+                m = make_float2(m.x + c * b[j].x, m.y + c * b[j].y);
+                e = make_float2(e.x + sn * d[j].x, e.y + sn * d[j].y);
+            }
+            // i * e = (-e.y, e.x)
+            v[k] = make_float2(m.x - e.y, m.y + e.x);
+            v[R - k] = make_float2(m.x + e.y, m.y - e.x);
+        }
+    }
+};
+template <int DIR> struct Dft<11, DIR> : DftOddPrime<11, DIR> {};
+template <int DIR> struct Dft<13, DIR> : DftOddPrime<13, DIR> {};
+
 // Composite R = A*B by in-register Cooley-Tukey:
 //   X[k1 + A*k2] = sum_n2 W_R^{n2 k1} [sum_n1 x[B*n1 + n2] W_A^{n1 k1}] W_B^{n2 k2}
 template <int A, int B, int DIR> struct DftSplit {
@@ -255,6 +294,8 @@ template <int DIR> struct Dft<32, DIR> : DftSplit<4, 8, DIR> {};
 template <int DIR> struct Dft<40, DIR> : DftSplit<8, 5, DIR> {};
 template <int DIR> struct Dft<48, DIR> : DftSplit<16, 3, DIR> {};
 template <int DIR> struct Dft<56, DIR> : DftSplit<8, 7, DIR> {};
+template <int DIR> struct Dft<9, DIR> : DftSplit<3, 3, DIR> {};
+template <int DIR> struct Dft<15, DIR> : DftSplit<3, 5, DIR> {};
 
 // --------------------------------------------------------------- FFT plans
 // Plan<N>: radices R[0..NP) (product N), threads per sequence T, P = N/T.
@@ -300,6 +341,21 @@ template <> struct Plan<2560> : PlanDef<2560, 64, RadixList<8, 8, 8, 5>> {};
 template <> struct Plan<3072> : PlanDef<3072, 64, RadixList<16, 16, 12>> {};
 template <> struct Plan<3584> : PlanDef<3584, 64, RadixList<8, 8, 8, 7>> {};
 template <> struct Plan<4096> : PlanDef<4096, 256, RadixList<16, 16, 16>> {};
+// Extents above 4096 (snapTransformSize gives multiples of 512 there): row
+// lengths N = fftW / 2 of 2304 ... 3840 and column lengths 4608 ... 8192.
+// Radices 9, 11, 13 and 15 go last; every radix divides P.
+template <> struct Plan<2304> : PlanDef<2304, 64, RadixList<4, 4, 4, 4, 9>> {};
+template <> struct Plan<2816> : PlanDef<2816, 64, RadixList<4, 4, 4, 4, 11>> {};
+template <> struct Plan<3328> : PlanDef<3328, 64, RadixList<4, 4, 4, 4, 13>> {};
+template <> struct Plan<3840> : PlanDef<3840, 64, RadixList<4, 4, 4, 4, 15>> {};
+template <> struct Plan<4608> : PlanDef<4608, 128, RadixList<4, 4, 4, 4, 2, 9>> {};
+template <> struct Plan<5120> : PlanDef<5120, 256, RadixList<4, 4, 4, 4, 4, 5>> {};
+template <> struct Plan<5632> : PlanDef<5632, 128, RadixList<4, 4, 4, 4, 2, 11>> {};
+template <> struct Plan<6144> : PlanDef<6144, 256, RadixList<8, 8, 8, 12>> {};
+template <> struct Plan<6656> : PlanDef<6656, 128, RadixList<4, 4, 4, 4, 2, 13>> {};
+template <> struct Plan<7168> : PlanDef<7168, 256, RadixList<4, 4, 4, 4, 4, 7>> {};
+template <> struct Plan<7680> : PlanDef<7680, 128, RadixList<4, 4, 4, 4, 2, 15>> {};
+template <> struct Plan<8192> : PlanDef<8192, 256, RadixList<16, 16, 32>> {};
 
 // Alternative plans for the extents of the benchmark workloads, selected at
 // run time per kernel kind (fewer registers per thread at the price of more

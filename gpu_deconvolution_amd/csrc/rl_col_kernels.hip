@@ -37,8 +37,9 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(
     col_tile<PL>(g, Xin + p * xstride, Xout + p * xstride, K, grp, lds, ksign);
 }
 
-#define FMVD_COL_SIZES(X) \
-    X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(3072) X(3584) X(4096)
+#define FMVD_COL_SIZES(X)                                                                               \
+    X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(3072) X(3584) X(4096) X(4608) \
+    X(5120) X(5632) X(6144) X(6656) X(7168) X(7680) X(8192)
 
 bool col_size_supported(int H)
 {

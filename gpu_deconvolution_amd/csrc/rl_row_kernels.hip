@@ -188,7 +188,9 @@ void launch_weight_normalize(float* out, const float* w, const float* sum, long 
 }
 
 // ------------------------------------------------------------- dispatchers
-#define FMVD_ROW_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(768) X(1024) X(1280) X(1536) X(1792) X(2048)
+#define FMVD_ROW_SIZES(X)                                                                              \
+    X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(768) X(1024) X(1280) X(1536) X(1792) X(2048) X(2304) \
+    X(2560) X(2816) X(3072) X(3328) X(3584) X(3840) X(4096)
 
 bool row_size_supported(int N)
 {

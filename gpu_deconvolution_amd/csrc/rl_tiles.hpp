@@ -42,8 +42,16 @@ template <typename PL_> struct RowCfg {
     static constexpr int T = PL::T;
     static constexpr int P = PL::P;
     // rows per workgroup: the plan's choice, else 256 threads where possible
-    // with at least 8 rows so the transposed loads/stores move 64-byte runs
-    static constexpr int G = PL::G ? PL::G : ((256 / T) > 16 ? 16 : ((256 / T) < 8 ? 8 : (256 / T)));
+    // with at least 8 rows so the transposed loads/stores move 64-byte runs;
+    // halved while the tile would exceed the 160 KiB of LDS a workgroup may
+    // hold or 1024 threads (long rows: 4 rows of N >= 2304)
+    static constexpr int G0 = PL::G ? PL::G : ((256 / T) > 16 ? 16 : ((256 / T) < 8 ? 8 : (256 / T)));
+    static constexpr int fit_rows(int g)
+    {
+        while (g > 1 && ((long)g * row_pitch(lds_len<N>(), g) * 8 > 160L * 1024 || g * T > 1024)) g /= 2;
+        return g;
+    }
+    static constexpr int G = fit_rows(G0);
     static constexpr int L = row_pitch(lds_len<N>(), G);
     static constexpr int THREADS = G * T;
     static constexpr int RFI = first_radix<PL, +1>();  // layout on entry of the inverse

@@ -138,7 +138,9 @@ int main()
 {
     CHECK_ALL(8) CHECK_ALL(16) CHECK_ALL(32) CHECK_ALL(64) CHECK_ALL(128) CHECK_ALL(256) CHECK_ALL(512)
     CHECK_ALL(768) CHECK_ALL(1024) CHECK_ALL(1280) CHECK_ALL(1536) CHECK_ALL(1792) CHECK_ALL(2048)
-    CHECK_ALL(2560) CHECK_ALL(3072) CHECK_ALL(3584) CHECK_ALL(4096)
+    CHECK_ALL(2560) CHECK_ALL(3072) CHECK_ALL(3584) CHECK_ALL(4096) CHECK_ALL(2304) CHECK_ALL(2816)
+    CHECK_ALL(3328) CHECK_ALL(3840) CHECK_ALL(4608) CHECK_ALL(5120) CHECK_ALL(5632) CHECK_ALL(6144)
+    CHECK_ALL(6656) CHECK_ALL(7168) CHECK_ALL(7680) CHECK_ALL(8192)
     printf("%s: %d plan(s) failed\n", failures ? "FAIL" : "PASS", failures);
     return failures ? 1 : 0;
 }

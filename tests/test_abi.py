@@ -61,8 +61,13 @@ def test_idivup_and_snap_match_oracle(fm, orc):
 def test_supported_extents_cover_workloads(fm):
     for h, w in [(512, 512), (1536, 1536), (2560, 2560), (256, 256), (1024, 1024), (256, 1024), (512, 256)]:
         assert fm.fft_extent_supported(h, w)
-    assert not fm.fft_extent_supported(4608, 4608)
-    assert not fm.fft_extent_supported(512, 5120)
+    # every extent snapTransformSize produces from 32 to 8192, on either axis
+    sizes = sorted({fm.snap_transform_size(n) for n in range(17, 8193)})
+    assert sizes[0] == 32 and sizes[-1] == 8192 and 4608 in sizes and 7680 in sizes
+    for n in sizes:
+        assert fm.fft_extent_supported(n, 512) and fm.fft_extent_supported(512, n), n
+    assert not fm.fft_extent_supported(8704, 512)
+    assert not fm.fft_extent_supported(512, 8704)
 
 
 def _kernels(v=2, k=5):
@@ -72,7 +77,7 @@ def _kernels(v=2, k=5):
 @pytest.mark.parametrize("kwargs,msg", [
     (dict(data_h=0, data_w=10), "invalid plan dimensions"),
     (dict(data_h=10, data_w=10, psf_type=7), "invalid PSF type"),
-    (dict(data_h=10, data_w=5000), "unsupported FFT extent"),
+    (dict(data_h=10, data_w=9000), "unsupported FFT extent"),
 ])
 def test_plan_validation_reports_through_handler(fm, kwargs, msg):
     h, w = max(kwargs["data_h"], 1), max(kwargs["data_w"], 1)

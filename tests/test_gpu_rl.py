@@ -64,6 +64,25 @@ def test_rl_parity(fm, orc, case):
     print(f"  fft={info['fft_h']}x{info['fft_w']} relL2={r:.2e} boundary flips={flips}", flush=True)
 
 
+# Extents above 4096 (snapTransformSize's multiples of 512, fmvd_deconvolve.cpp:28-51):
+# every row length N = fftW/2 of 2304 ... 4096 and every column length 4608 ...
+# 8192, on thin planes so the oracle stays fast.
+LARGE_CASES = [(50, w) for w in (4560, 5000, 5500, 6000, 6500, 7000, 7500, 8000)] + \
+              [(h, 50) for h in (4560, 5000, 5500, 6000, 6500, 7000, 7500, 8000)]
+
+
+@pytest.mark.parametrize("case", LARGE_CASES, ids=lambda c: f"{c[0]}x{c[1]}")
+def test_rl_parity_large_extents(fm, orc, case):
+    dh, dw = case
+    print(f"\n[rl parity large] {dh}x{dw}", flush=True)
+    data, kernels, weights = make_case(1, 2, dh, dw, 41, 41, 16, seed=dh + 3 * dw, beads=30)
+    out, est, info = run_gpu(fm, data, kernels, weights, 0, 2, 16)
+    assert max(info["fft_h"], info["fft_w"]) > 4096
+    out_o, est_o = orc.rl_planes(data, weights, kernels.copy(), 0, 2, threads=8)
+    r, flips = check_rl(out, est, out_o, est_o)
+    print(f"  fft={info['fft_h']}x{info['fft_w']} relL2={r:.2e} boundary flips={flips}", flush=True)
+
+
 VARIANT_CASES = [
     (1, 2, 230, 250, 31, 31),     # 512 x 512
     (1, 2, 600, 500, 41, 41),     # 1024 x 1024
