@@ -81,6 +81,8 @@ def lib() -> C.CDLL:
         "fmvd_set_chunk_planes_cuda_8": (i, [vp, i]),
         "fmvd_deconvolve_volume_device_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_deconvolve_volume_device_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
+        "fmvd_deconvolve_volume_device_noalloc_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
+        "fmvd_deconvolve_volume_device_noalloc_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_set_lookahead_cuda_16": (i, [vp, i]),
         "fmvd_set_lookahead_cuda_8": (i, [vp, i]),
         "fmvd_set_weights_device_cuda_16": (i, [vp, C.POINTER(vp), vp]),
@@ -329,6 +331,20 @@ class Deconvolver:
         check()
         if rc != 0:
             raise FmvdError("volume deconvolution failed")
+
+    def run_device_noalloc(self, view_ptrs: Sequence[int], nplanes: int, iterations: int, out_ptr: int = 0,
+                           est_ptr: int = 0, init_ptr: int = 0, stream: int = 0) -> None:
+        """run_device through the allocation-free entry point (work buffers in
+        the library's fixed device scratch)."""
+        L = lib()
+        fn = (L.fmvd_deconvolve_volume_device_noalloc_cuda_16 if self.bits == 16
+              else L.fmvd_deconvolve_volume_device_noalloc_cuda_8)
+        arr = (C.c_void_p * self.nviews)(*[C.c_void_p(int(p)) for p in view_ptrs])
+        rc = fn(self.plan, arr, int(nplanes), int(iterations), C.c_void_p(init_ptr or None),
+                C.c_void_p(out_ptr or None), C.c_void_p(est_ptr or None), C.c_void_p(stream or None))
+        check()
+        if rc != 0:
+            raise FmvdError("allocation-free volume deconvolution failed")
 
     def convergence(self, view_ptrs: Sequence[int], nplanes: int, iterations: int, ref_ptr: int,
                     init_ptr: int = 0, stream: int = 0) -> np.ndarray:

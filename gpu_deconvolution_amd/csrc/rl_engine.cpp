@@ -37,6 +37,7 @@
 #include "rl_launch.hpp"
 #include "timing.hpp"
 #include "transform.hpp"
+#include "noalloc_scratch.hpp"
 
 namespace fmvd {
 
@@ -126,8 +127,16 @@ template <typename S> struct Plan {
         return (int)std::max<size_t>(nb, (size_t)std::max(1, nStreams));
     }
 
+    // Work buffers of the running call: the plan's own (XE, ES), or, in the
+    // allocation-free entry point, slices of the device scratch array.
+    float2* workX = nullptr;
+    float* workE = nullptr;
+    float2* xbase() const { return workX ? workX : XE.as<float2>(); }
+    float* ebase() const { return workX ? workE : ES.as<float>(); }
+
     void ensure_work(int planes)
     {
+        if (workX) return;  // scratch slices are sized by the caller
         XE.alloc(xplane() * sizeof(float2) * (size_t)planes);
         ES.alloc(eplane() * sizeof(float) * (size_t)planes);
     }
@@ -248,7 +257,7 @@ template <typename S> struct Plan {
     {
         ensure_work(np);
         const int c = std::max(1, std::min(np, chunk_planes()));
-        const int lanes = errAcc ? 1 : (int)std::max<size_t>(1, std::min<size_t>(2, env_bytes("FMVD_LANES", 1)));
+        const int lanes = (errAcc || workX) ? 1 : (int)std::max<size_t>(1, std::min<size_t>(2, env_bytes("FMVD_LANES", 1)));
         if (lanes > 1 && np > c) {
             if (!lane2) {
                 lane2.reset(new Stream());
@@ -260,7 +269,7 @@ template <typename S> struct Plan {
         }
         for (int start = 0, i = 0; start < np; start += c, ++i)
             run_chunk(views, dstride, start, std::min(c, np - start), iterations, init, istride, out, est, ostride,
-                      (lanes > 1 && (i & 1)) ? lane2->s : s, refOffset, i % kTimingStride == 0);
+                      (lanes > 1 && (i & 1)) ? lane2->s : s, refOffset, !workX && i % kTimingStride == 0);
         if (lanes > 1 && np > c) {
             FMVD_HIP(hipEventRecord(laneJoin->e, lane2->s));
             FMVD_HIP(hipStreamWaitEvent(s, laneJoin->e, 0));
@@ -279,9 +288,9 @@ template <typename S> struct Plan {
         const ColGeom cg = col_geom();
         const long xs = (long)xplane();
         const long esd = (long)eplane();
-        float2* xe = XE.as<float2>() + (size_t)start * xs;
+        float2* xe = xbase() + (size_t)start * xs;
         float2* xt = xe;  // passes A-C write the spectrum in place
-        float* es = ES.as<float>() + (size_t)start * esd;
+        float* es = ebase() + (size_t)start * esd;
         const float* ref = errRef ? errRef + refOffset + (size_t)start * plane() : nullptr;
         if (init) init += (size_t)start * istride;
         if (out) out += (size_t)start * ostride;
@@ -443,6 +452,45 @@ template <typename S> struct Plan {
             run_batch(vv.data(), (long)pl, n, iterations, init ? init + (size_t)start * pl : nullptr,
                       (long)pl, out ? out + (size_t)start * pl : nullptr,
                       est ? est + (size_t)start * pl : nullptr, (long)pl, s, (long)start * (long)pl);
+        }
+    }
+
+    // run_volume with the work buffers in the device scratch array instead of
+    // the plan's own: no device memory is allocated or freed.  Planes run in
+    // batches of as many as the scratch holds.  Every call on one device
+    // shares that device's scratch, so calls that overlap on one GPU race.
+    static constexpr int kNoAllocMaxViews = 64;
+    void run_volume_noalloc(const S* const* dviews, int nplanes, int iterations, const float* init, S* out,
+                            float* est, hipStream_t s)
+    {
+        if (nplanes < 0 || iterations < 0) throw_msg("negative plane or iteration count");
+        if (!dviews) throw_msg("d_views must not be NULL");
+        if (nViews > kNoAllocMaxViews) throw_msg("the allocation-free entry point takes at most 64 views");
+        const size_t xb = xplane() * sizeof(float2), eb = eplane() * sizeof(float);
+        const size_t cap = kNoAllocScratchBytes / (xb + eb);
+        if (cap < 1) {
+            char msg[200];
+            snprintf(msg, sizeof msg,
+                     "FFT extent %d x %d needs %zu bytes of scratch per plane; the allocation-free entry point "
+                     "has %zu", fftH, fftW, xb + eb, kNoAllocScratchBytes);
+            throw_msg(msg);
+        }
+        unsigned char* base = static_cast<unsigned char*>(noalloc_scratch());
+        const int nb = (int)std::min<size_t>((size_t)batch_planes(), cap);
+        struct Restore {
+            Plan* p;
+            ~Restore() { p->workX = nullptr; p->workE = nullptr; }
+        } restore{this};
+        workX = reinterpret_cast<float2*>(base);
+        workE = reinterpret_cast<float*>(base + (size_t)nb * xb);
+        const size_t pl = plane();
+        const S* vv[kNoAllocMaxViews];
+        for (int start = 0; start < nplanes; start += nb) {
+            const int n = std::min(nb, nplanes - start);
+            for (int v = 0; v < nViews; ++v) vv[v] = dviews[v] + (size_t)start * pl;
+            run_batch(vv, (long)pl, n, iterations, init ? init + (size_t)start * pl : nullptr, (long)pl,
+                      out ? out + (size_t)start * pl : nullptr, est ? est + (size_t)start * pl : nullptr, (long)pl,
+                      s, (long)start * (long)pl);
         }
     }
 
@@ -856,6 +904,32 @@ int fmvd_deconvolve_volume_device_cuda_16(const struct fmvd_plan_cuda_16* plan,
         if (!plan) fmvd::throw_msg("NULL plan");
         const_cast<fmvd_plan_cuda_16*>(plan)->impl.run_volume(d_views, nPlanes, iterations, d_init, d_out,
                                                               d_estimate, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_deconvolve_volume_device_noalloc_cuda_16(const struct fmvd_plan_cuda_16* plan,
+                                                  const unsigned short* const* d_views, int nPlanes, int iterations,
+                                                  const float* d_init, unsigned short* d_out, float* d_estimate,
+                                                  void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_16*>(plan)->impl.run_volume_noalloc(d_views, nPlanes, iterations, d_init, d_out,
+                                                                      d_estimate, (hipStream_t)stream);
+        return 0;
+    })
+}
+
+int fmvd_deconvolve_volume_device_noalloc_cuda_8(const struct fmvd_plan_cuda_8* plan,
+                                                 const unsigned char* const* d_views, int nPlanes, int iterations,
+                                                 const float* d_init, unsigned char* d_out, float* d_estimate,
+                                                 void* stream)
+{
+    FMVD_API_GUARD(-1, {
+        if (!plan) fmvd::throw_msg("NULL plan");
+        const_cast<fmvd_plan_cuda_8*>(plan)->impl.run_volume_noalloc(d_views, nPlanes, iterations, d_init, d_out,
+                                                                     d_estimate, (hipStream_t)stream);
         return 0;
     })
 }

@@ -172,6 +172,28 @@ int fmvd_deconvolve_volume_device_cuda_8(const struct fmvd_plan_cuda_8 *plan,
                                          int iterations, const float *d_init, unsigned char *d_out,
                                          float *d_estimate, void *stream);
 
+/* Allocation-free variant of fmvd_deconvolve_volume_device_cuda_* (same
+ * arguments, same results bit for bit).  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  It calls no hipMalloc/hipFree or other
+ * allocating routine: the work buffers (spectrum and estimate of each
+ * plane) live in a fixed 128 MiB device array that every GPU the library
+ * runs on holds from module load until the process ends, and the planes
+ * run in batches of as many as it holds (2 at FFT 2560 x 2560, 6 at
+ * 1536 x 1536).  Plans whose single plane needs more (FFT extents above
+ * about 4096 x 4096) are rejected with an error.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU race; the array
+ * needs no clearing (each plane's buffers are written before they are
+ * read).  The plan must have been created beforehand. */
+int fmvd_deconvolve_volume_device_noalloc_cuda_16(const struct fmvd_plan_cuda_16 *plan,
+                                                  const unsigned short *const *d_views, int nPlanes,
+                                                  int iterations, const float *d_init, unsigned short *d_out,
+                                                  float *d_estimate, void *stream);
+int fmvd_deconvolve_volume_device_noalloc_cuda_8(const struct fmvd_plan_cuda_8 *plan,
+                                                 const unsigned char *const *d_views, int nPlanes,
+                                                 int iterations, const float *d_init, unsigned char *d_out,
+                                                 float *d_estimate, void *stream);
+
 /* Autotuner: times every compiled FFT-plan variant of each device kernel
  * kind (0 column pass, 1 ratio row pass, 2 update row pass, 3 warm-start
  * ingest, 4 egress) on this plan's buffers over `planes` planes (0 = 32)
