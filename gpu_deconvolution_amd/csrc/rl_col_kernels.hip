@@ -74,8 +74,8 @@ static void launch_col_plan(const ColGeom& g, const float2* Xin, float2* Xout, l
                             int nplanes, hipStream_t s, bool conj)
 {
     using C = ColCfg<PL>;
-    if (g.N % C::SQ) throw_msg("row count not divisible by column group");
-    dim3 grid((unsigned)(g.N / C::SQ * nplanes));
+    if (g.N % C::SQ && C::T > 64) throw_msg("row count not divisible by column group");
+    dim3 grid((unsigned)((g.N + C::SQ - 1) / C::SQ * nplanes));
     launch(col_conv_kernel<PL>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes, conj ? -1.0f : 1.0f);
 }
 

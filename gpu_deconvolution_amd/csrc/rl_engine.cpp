@@ -38,6 +38,7 @@
 #include "timing.hpp"
 #include "transform.hpp"
 #include "noalloc_scratch.hpp"
+#include "autotune_cache.hpp"
 
 namespace fmvd {
 
@@ -159,7 +160,7 @@ template <typename S> struct Plan {
         fftH = snap_transform_size(dataH + kernelH - 1);
         fftW = snap_transform_size(dataW + kernelW - 1);
         N = fftW / 2;
-        if (!row_size_supported(N) || !col_size_supported(fftH) || fftW < 32 || fftH < 16) {
+        if (!row_size_supported(N) || !col_size_supported(fftH) || fftW < 16 || fftH < 16) {
             char msg[160];
             snprintf(msg, sizeof msg, "unsupported FFT extent %d x %d (data %d x %d, kernel %d x %d)",
                      fftH, fftW, dataH, dataW, kernelH, kernelW);
@@ -331,10 +332,57 @@ template <typename S> struct Plan {
     // this plan's own buffers (planes planes, synthetic content) and keeps the
     // fastest.  times[kind*3 + variant] receives the mean launch time in ms
     // (negative where the variant does not exist).
+    // Key of this plan's tuning in the autotuner cache file: device, extents,
+    // sample type, kernel-hat form and the planes the launches cover.
+    std::string tune_key(int planes) const
+    {
+        hipDeviceProp_t prop;
+        FMVD_HIP(hipGetDeviceProperties(&prop, device));
+        char key[256];
+        snprintf(key, sizeof key, "%s|%d|%dx%d|u%d|c%d|p%d", prop.gcnArchName, cus, fftH, fftW, (int)(8 * sizeof(S)),
+                 khatConj ? 1 : 0, planes);
+        for (char* c = key; *c; ++c)
+            if (*c == ' ') *c = '_';
+        return key;
+    }
+
     void autotune(int planes, float* times)
     {
         FMVD_HIP(hipSetDevice(device));
         planes = std::max(1, std::min({planes > 0 ? planes : 32, batch_planes(), chunk_planes()}));
+        const std::string key = tune_key(planes);
+        TuneRecord rec;
+        if (autotune_cache_find(key, rec)) {
+            int v[5];
+            std::copy(rec.chosen, rec.chosen + 5, v);
+            if (valid_variants(v)) {
+                var = Variants{v[0], v[1], v[2], v[3], v[4]};
+                if (times) std::copy(rec.times, rec.times + 15, times);
+                return;
+            }
+        }
+        float measured[15];
+        autotune_measure(planes, measured);
+        rec.chosen[0] = var.column;
+        rec.chosen[1] = var.ratio;
+        rec.chosen[2] = var.update;
+        rec.chosen[3] = var.ingest;
+        rec.chosen[4] = var.egress;
+        std::copy(measured, measured + 15, rec.times);
+        autotune_cache_store(key, rec);
+        if (times) std::copy(measured, measured + 15, times);
+    }
+
+    bool valid_variants(const int* v) const
+    {
+        const int nc = col_variants(fftH), nr = row_variants(N);
+        for (int k = 0; k < 4; ++k)
+            if (v[k] < 0 || v[k] >= (k == 0 ? nc : nr)) return false;
+        return v[4] == 0;
+    }
+
+    void autotune_measure(int planes, float* times)
+    {
         ensure_work(planes);
         const size_t pl = plane();
         DeviceBuffer data, out, e0;
@@ -726,10 +774,7 @@ template <typename PlanT> static int autotune_plan(PlanT* plan, int planes, int*
 template <typename PlanT> static int set_variants(PlanT* plan, const int* v)
 {
     if (!plan || !v) return -1;
-    const int nc = fmvd::col_variants(plan->impl.fftH), nr = fmvd::row_variants(plan->impl.N);
-    for (int k = 0; k < 4; ++k)
-        if (v[k] < 0 || v[k] >= (k == 0 ? nc : nr)) return -1;
-    if (v[4] != 0) return -1;  // the egress kernel has no FFT plan
+    if (!plan->impl.valid_variants(v)) return -1;  // (the egress kernel has no FFT plan: v[4] must be 0)
     fmvd::Variants& x = plan->impl.var;
     x.column = v[0];
     x.ratio = v[1];
@@ -988,7 +1033,7 @@ int fmvd_snap_transform_size(int n) { return fmvd::snap_transform_size(n); }
 
 int fmvd_fft_extent_supported(int fftH, int fftW)
 {
-    return fftW >= 32 && fftH >= 16 && (fftW % 2 == 0) && fmvd::row_size_supported(fftW / 2) &&
+    return fftW >= 16 && fftH >= 16 && (fftW % 2 == 0) && fmvd::row_size_supported(fftW / 2) &&
            fmvd::col_size_supported(fftH);
 }
 

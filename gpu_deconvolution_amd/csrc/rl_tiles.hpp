@@ -467,6 +467,10 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
     // wave-uniform (a sequence never spans two wavefronts here): keep it scalar
     // so the loads below address from one SGPR base with 32-bit lane offsets
     const int kx = C::T % 64 == 0 ? __builtin_amdgcn_readfirstlane(grp * C::SQ + seq) : grp * C::SQ + seq;
+    // only a group of 16 sequences of at most 16 threads (columns up to 256)
+    // can overhang N (= 8 at fftW = 16); such sequences synchronise within
+    // their wavefront, so the overhanging ones just leave
+    if (kx >= g.N) return;
     const float2* xin = Xin + (unsigned)xpos(kx, 0, H);  // stride 2 along y
     float2* row = lds + seq * C::L;
 

@@ -30,16 +30,58 @@ struct Mat12 {
     double m[12];
 };
 
-// One output sample at (x, y, z); fetch(ix, iy, iz) returns the texel or 0
-// outside the volume.  Fixed FP64 operation order, no contraction.
-template <typename S, typename Fetch>
-__device__ __forceinline__ S transform_sample(int w, int h, int d, const Mat12& M, int x, int y, int z, Fetch fetch)
+// Two horizontally adjacent samples (x, x + 1) of one source row as one
+// 4-byte (16-bit samples) or 2-byte (8-bit samples) load; x need not be even.
+template <typename S>
+__device__ __forceinline__ void load_pair(const S* row, int x, double& t0, double& t1)
+{
+    if constexpr (sizeof(S) == 2) {
+        uint32_t u;
+        __builtin_memcpy(&u, row + x, 4);
+        t0 = (double)(u & 0xffffu);
+        t1 = (double)(u >> 16);
+    } else {
+        uint16_t u;
+        __builtin_memcpy(&u, row + x, 2);
+        t0 = (double)(u & 0xffu);
+        t1 = (double)(u >> 8);
+    }
+}
+
+// Samples (ix, iy, iz) and (ix + 1, iy, iz), zero outside the volume.
+template <typename S>
+__device__ __forceinline__ void fetch_pair(const S* __restrict__ src, int w, int h, int d, int ix, int iy, int iz,
+                                           double& t0, double& t1)
+{
+    if (iy < 0 || iy >= h || iz < 0 || iz >= d) {
+        t0 = t1 = 0.0;
+        return;
+    }
+    const S* row = src + ((size_t)iz * h + iy) * w;
+    if (ix >= 0 && ix + 1 < w) {
+        load_pair<S>(row, ix, t0, t1);
+    } else {
+        t0 = (ix >= 0 && ix < w) ? (double)row[ix] : 0.0;
+        t1 = (ix + 1 >= 0 && ix + 1 < w) ? (double)row[ix + 1] : 0.0;
+    }
+}
+
+// One output sample at plane z of target column (x, y); axy[a] holds
+// m[4a] x + m[4a+1] y (the first sum of the fixed operation order).  FP64
+// with no contraction:
+//   r_a = ((m[4a] x + m[4a+1] y) + m[4a+2] z) + m[4a+3], i = floor(r), f = r - i,
+//   nested lerps v (1 - f) + v' f in x, then y, then z (texels outside -> 0),
+//   out = clamp(floor(interp / (2^B-1) * 2^B + 0.5), 0, 2^B-1)
+// (fmvd_transform.impl.cu:41-53 with FP64 coordinates and weights).
+template <typename S>
+__device__ __forceinline__ S transform_sample(const S* __restrict__ src, int w, int h, int d, const Mat12& M,
+                                              const double* axy, int z)
 {
 #pragma clang fp contract(off)
-    const double xd = (double)x, yd = (double)y, zd = (double)z;
-    const double rx = ((M.m[0] * xd + M.m[1] * yd) + M.m[2] * zd) + M.m[3];
-    const double ry = ((M.m[4] * xd + M.m[5] * yd) + M.m[6] * zd) + M.m[7];
-    const double rz = ((M.m[8] * xd + M.m[9] * yd) + M.m[10] * zd) + M.m[11];
+    const double zd = (double)z;
+    const double rx = (axy[0] + M.m[2] * zd) + M.m[3];
+    const double ry = (axy[1] + M.m[6] * zd) + M.m[7];
+    const double rz = (axy[2] + M.m[10] * zd) + M.m[11];
     double val = 0.0;
     if (rx > -2.0 && ry > -2.0 && rz > -2.0 && rx < (double)w + 1.0 && ry < (double)h + 1.0 &&
         rz < (double)d + 1.0) {
@@ -47,10 +89,15 @@ __device__ __forceinline__ S transform_sample(int w, int h, int d, const Mat12& 
         const double ax = rx - fx, ay = ry - fy, az = rz - fz;
         const double bx = 1.0 - ax, by = 1.0 - ay, bz = 1.0 - az;
         const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-        const double c00 = fetch(ix, iy, iz) * bx + fetch(ix + 1, iy, iz) * ax;
-        const double c10 = fetch(ix, iy + 1, iz) * bx + fetch(ix + 1, iy + 1, iz) * ax;
-        const double c01 = fetch(ix, iy, iz + 1) * bx + fetch(ix + 1, iy, iz + 1) * ax;
-        const double c11 = fetch(ix, iy + 1, iz + 1) * bx + fetch(ix + 1, iy + 1, iz + 1) * ax;
+        double t000, t100, t010, t110, t001, t101, t011, t111;
+        fetch_pair<S>(src, w, h, d, ix, iy, iz, t000, t100);
+        fetch_pair<S>(src, w, h, d, ix, iy + 1, iz, t010, t110);
+        fetch_pair<S>(src, w, h, d, ix, iy, iz + 1, t001, t101);
+        fetch_pair<S>(src, w, h, d, ix, iy + 1, iz + 1, t011, t111);
+        const double c00 = t000 * bx + t100 * ax;
+        const double c10 = t010 * bx + t110 * ax;
+        const double c01 = t001 * bx + t101 * ax;
+        const double c11 = t011 * bx + t111 * ax;
         const double c0 = c00 * by + c10 * ay;
         const double c1 = c01 * by + c11 * ay;
         val = c0 * bz + c1 * az;
@@ -72,113 +119,48 @@ __device__ __forceinline__ S transform_sample(int w, int h, int d, const Mat12& 
     return (S)0;
 }
 
-// Output tile of a workgroup: 16 x 16 x 8 voxels, thread (tx, ty) computes
-// the 8 voxels of its column.  The tile's source footprint is the bounding
-// box of its 8 corner images (an affine map takes its extremes at corners),
-// widened by one texel for rounding and clamped to [-3, extent + 2], which
-// holds every texel an in-range sample can read.  When the box fits the
-// LDS brick it is staged with row-contiguous (coalesced) loads, zero outside
-// the volume, and all trilinear fetches come from LDS; otherwise the
-// workgroup gathers from global memory.  The brick (dynamic LDS, `cap`
-// texels) is sized on the host from the map: every tile's box fits it, and a
-// smaller brick lets more workgroups share a CU.
-constexpr int kTX = 16, kTY = 16, kTZ = 8;
-constexpr int kBrickMaxBytes = 48 * 1024;
+// A thread computes kGZ consecutive target planes of one (x, y) column; a
+// workgroup holds 64 x 4 columns.  The trilinear fetches read the source
+// straight from global memory (two samples per load).  Under a rotation the
+// 64 lanes of a wavefront walk diagonally through the source, so a source
+// cache line is used by outputs that are neighbours in 3D, not along one
+// target row: workgroups are therefore taken in supertiles of kSX x kSY x kSZ
+// workgroups (128 x 32 x 32 target voxels, about 1.5 MB of source), numbered
+// x fastest, then y, then z, with each XCD given a contiguous range, so the
+// supertiles in flight on one XCD keep their source rows in its L2.
+constexpr int kGX = 64, kGY = 4, kGZ = 8;
+constexpr unsigned kSX = 2, kSY = 8, kSZ = 4, kSuper = kSX * kSY * kSZ;
+
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks)
+{
+    const unsigned q = nblocks >> 3, rem = nblocks & 7u;
+    const unsigned x = bid & 7u, k = bid >> 3;
+    return x * q + (x < rem ? x : rem) + k;
+}
 
 template <typename S>
 __global__ __launch_bounds__(256) void transform_kernel(const S* __restrict__ src, int w, int h, int d,
                                                         S* __restrict__ dst, int tw, int th, int z0, int nz,
-                                                        Mat12 M, int cap)
+                                                        Mat12 M, unsigned nbx, unsigned nby, unsigned nsx,
+                                                        unsigned nsy)
 {
 #pragma clang fp contract(off)
-    extern __shared__ __align__(16) unsigned char brick_lds[];
-    S* brick = reinterpret_cast<S*>(brick_lds);
-    const int CAP = cap;
-    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, zt0 = z0 + blockIdx.z * kTZ;
-    const int x1 = min(x0 + kTX, tw) - 1, y1 = min(y0 + kTY, th) - 1, zt1 = min(zt0 + kTZ, z0 + nz) - 1;
-    // bounding box of the tile's source footprint (block-uniform)
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int c = 0; c < 8; ++c) {
-        const double xd = (double)((c & 1) ? x1 : x0), yd = (double)((c & 2) ? y1 : y0),
-                     zd = (double)((c & 4) ? zt1 : zt0);
-        for (int a = 0; a < 3; ++a) {
-            const double r = ((M.m[4 * a] * xd + M.m[4 * a + 1] * yd) + M.m[4 * a + 2] * zd) + M.m[4 * a + 3];
-            lo[a] = fmin(lo[a], r);
-            hi[a] = fmax(hi[a], r);
-        }
-    }
-    const int ext[3] = {w, h, d};
-    int b0[3], bn[3];
-    bool outside = false;
-    long cells = 1;
-    for (int a = 0; a < 3; ++a) {
-        // (a margin keeps interior points that round past a corner value in the staged path; NaN -> outside)
-        if (!(hi[a] > -2.0 - 1e-6) || !(lo[a] < (double)ext[a] + 1.0 + 1e-6)) outside = true;
-        const double l = fmax(floor(lo[a]) - 1.0, -3.0), u = fmin(floor(hi[a]) + 2.0, (double)ext[a] + 2.0);
-        b0[a] = outside ? 0 : (int)l;
-        bn[a] = outside ? 1 : max(1, (int)u - (int)l + 1);
-        cells *= bn[a];
-    }
-    const int x = x0 + tx, y = y0 + ty;
-    const bool active = x <= x1 && y <= y1;
-    if (outside) {  // every sample of the tile is outside the source: zeros
-        if (active)
-            for (int z = zt0; z <= zt1; ++z) dst[((long)(z - z0) * th + y) * tw + x] = (S)0;
-        return;
-    }
-    if (cells <= CAP) {
-        const int BX = bn[0], BY = bn[1];
-        // 16 texels per thread in flight before their LDS writes.  The brick
-        // index e splits into (x, y, z) with float reciprocals and one
-        // correction step (e < 2^16: the quotient is off by at most one).
-        constexpr int kU = 16;
-        const float inv_bx = 1.0f / (float)BX, inv_by = 1.0f / (float)BY;
-        for (int e0 = threadIdx.x; e0 < (int)cells; e0 += 256 * kU) {
-            S v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int e = e0 + 256 * u;
-                int q = (int)((float)e * inv_bx), ex = e - q * BX;
-                if (ex < 0) {
-                    --q;
-                    ex += BX;
-                } else if (ex >= BX) {
-                    ++q;
-                    ex -= BX;
-                }
-                int ez = (int)((float)q * inv_by), ey = q - ez * BY;
-                if (ey < 0) {
-                    --ez;
-                    ey += BY;
-                } else if (ey >= BY) {
-                    ++ez;
-                    ey -= BY;
-                }
-                const int gx = b0[0] + ex, gy = b0[1] + ey, gz = b0[2] + ez;
-                const bool in = e < (int)cells && gx >= 0 && gy >= 0 && gz >= 0 && gx < w && gy < h && gz < d;
-                v[u] = in ? src[((long)gz * h + gy) * w + gx] : (S)0;
-            }
-#pragma unroll
-            for (int u = 0; u < kU; ++u)
-                if (e0 + 256 * u < (int)cells) brick[e0 + 256 * u] = v[u];
-        }
-        __syncthreads();
-        if (!active) return;
-        auto fetch = [&](int ix, int iy, int iz) -> double {
-            return (double)brick[((iz - b0[2]) * BY + (iy - b0[1])) * BX + (ix - b0[0])];
-        };
-        for (int z = zt0; z <= zt1; ++z)
-            dst[((long)(z - z0) * th + y) * tw + x] = transform_sample<S>(w, h, d, M, x, y, z, fetch);
-    } else {
-        if (!active) return;
-        auto fetch = [&](int ix, int iy, int iz) -> double {
-            if (ix < 0 || iy < 0 || iz < 0 || ix >= w || iy >= h || iz >= d) return 0.0;
-            return (double)src[((long)iz * h + iy) * w + ix];
-        };
-        for (int z = zt0; z <= zt1; ++z)
-            dst[((long)(z - z0) * th + y) * tw + x] = transform_sample<S>(w, h, d, M, x, y, z, fetch);
-    }
+    const unsigned lid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned st = lid / kSuper, wi = lid % kSuper;
+    const unsigned bx = (st % nsx) * kSX + wi % kSX;
+    const unsigned by = ((st / nsx) % nsy) * kSY + (wi / kSX) % kSY;
+    const unsigned bz = (st / (nsx * nsy)) * kSZ + wi / (kSX * kSY);
+    if (bx >= nbx || by >= nby) return;
+    const int x = (int)bx * kGX + (int)(threadIdx.x % kGX);
+    const int y = (int)by * kGY + (int)(threadIdx.x / kGX);
+    if (x >= tw || y >= th) return;
+    const double xd = (double)x, yd = (double)y;
+    const double axy[3] = {M.m[0] * xd + M.m[1] * yd, M.m[4] * xd + M.m[5] * yd, M.m[8] * xd + M.m[9] * yd};
+    const int zb = z0 + (int)bz * kGZ, ze = min(zb + kGZ, z0 + nz);
+    if (zb >= ze) return;
+    S* out = dst + ((size_t)(zb - z0) * th + y) * tw + x;
+    const size_t pstride = (size_t)th * tw;
+    for (int z = zb; z < ze; ++z, out += pstride) *out = transform_sample<S>(src, w, h, d, M, axy, z);
 }
 
 __global__ __launch_bounds__(256) void mask_kernel(float* __restrict__ out, int z, int w, int h, int d,
@@ -221,23 +203,14 @@ static void launch_transform(const S* src, int w, int h, int d, S* dst, int tw, 
                              const Mat12& M, hipStream_t s)
 {
     if (nz <= 0 || tw <= 0 || th <= 0) return;
-    dim3 grid((unsigned)((tw + kTX - 1) / kTX), (unsigned)((th + kTY - 1) / kTY), (unsigned)((nz + kTZ - 1) / kTZ));
-    // Box of a tile's source footprint per axis: the corner images span
-    // sum_j |m_aj| (tile extent_j - 1); flooring and the one-texel widening
-    // on both sides add at most 4 cells (+1 for rounding of the span).
-    const int ext[3] = {kTX, kTY, kTZ};
-    double cells = 1.0;
-    for (int a = 0; a < 3; ++a) {
-        double span = 0.0;
-        for (int j = 0; j < 3; ++j) span += std::fabs(M.m[4 * a + j]) * (double)(ext[j] - 1);
-        cells *= std::ceil(span) + 5.0;
-    }
-    const double maxb = (double)kBrickMaxBytes;
-    const int bytes = cells * sizeof(S) >= maxb ? kBrickMaxBytes
-                                                : (int)((cells * sizeof(S) + 255.0) / 256.0) * 256;
+    const unsigned nbx = (unsigned)((tw + kGX - 1) / kGX), nby = (unsigned)((th + kGY - 1) / kGY);
+    const unsigned nbz = (unsigned)((nz + kGZ - 1) / kGZ);
+    const unsigned nsx = (nbx + kSX - 1) / kSX, nsy = (nby + kSY - 1) / kSY, nsz = (nbz + kSZ - 1) / kSZ;
+    const unsigned long long nblocks = (unsigned long long)nsx * nsy * nsz * kSuper;
+    if (nblocks > 0x7fffffffull) throw_msg("transform: target volume too large for one launch");
     TimedScope ts(kTimeTransform, s);
-    launch(transform_kernel<S>, grid, dim3(256), (unsigned)bytes, s, src, w, h, d, dst, tw, th, z0, nz, M,
-           bytes / (int)sizeof(S));
+    launch(transform_kernel<S>, dim3((unsigned)nblocks), dim3(kGX * kGY), 0, s, src, w, h, d, dst, tw, th, z0, nz,
+           M, nbx, nby, nsx, nsy);
     FMVD_LAUNCH_CHECK();
 }
 

@@ -62,8 +62,8 @@ def test_supported_extents_cover_workloads(fm):
     for h, w in [(512, 512), (1536, 1536), (2560, 2560), (256, 256), (1024, 1024), (256, 1024), (512, 256)]:
         assert fm.fft_extent_supported(h, w)
     # every extent snapTransformSize produces from 32 to 8192, on either axis
-    sizes = sorted({fm.snap_transform_size(n) for n in range(17, 8193)})
-    assert sizes[0] == 32 and sizes[-1] == 8192 and 4608 in sizes and 7680 in sizes
+    sizes = sorted({fm.snap_transform_size(n) for n in range(1, 8193)})
+    assert sizes[0] == 16 and sizes[-1] == 8192 and 4608 in sizes and 7680 in sizes
     for n in sizes:
         assert fm.fft_extent_supported(n, 512) and fm.fft_extent_supported(512, n), n
     assert not fm.fft_extent_supported(8704, 512)
@@ -134,3 +134,7 @@ def test_batch_rejects_bad_volumes_before_touching_the_device(fm):
     assert not L.fmvd_batch_create_cuda_16(1, arr)
     errs = fm.take_errors()
     assert errs and "invalid volume dimensions" in errs[0]
+
+
+def test_autotune_cache_line_format(fm):
+    assert fm.lib().fmvd_autotune_cache_selftest() == 0

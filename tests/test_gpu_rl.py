@@ -49,6 +49,9 @@ CASES = [
     (2, 2, 90, 100, 15, 12, 0, 8, 4),         # 128 x 128, odd x even kernel: explicit, 8-bit
     (2, 3, 101, 77, 15, 15, 0, 16, 5),        # 128 x 128, odd data width (per-sample pad map loads)
     (2, 2, 77, 131, 13, 17, 1, 8, 4),         # 128 x 256, odd width, 8-bit
+    (2, 2, 7, 6, 9, 9, 0, 16, 5),             # 16 x 16: the smallest extent (row length 8)
+    (2, 2, 300, 6, 9, 9, 0, 16, 5),           # 512 x 16
+    (2, 2, 6, 300, 9, 9, 2, 16, 5),           # 16 x 512
 ]
 
 
@@ -277,3 +280,23 @@ def test_convergence_trace_matches_oracle_estimates(fm, orc):
     assert errs[4] < 1e-5
     for a, b in zip(errs[:4], want[:4]):
         assert abs(a - b) <= 1e-4 * b + 1e-6
+
+
+def test_autotune_cache_file(fm, tmp_path):
+    """With a cache file set, the first autotune of a configuration times the
+    variants and records them; a second plan of the same configuration gets
+    the same picks from the file without timing (one line, unchanged)."""
+    path = tmp_path / "tune.txt"
+    data, kernels, weights = make_case(1, 2, 600, 500, 41, 41, 16, seed=4)
+    fm.lib().fmvd_set_autotune_cache(str(path).encode())
+    try:
+        with fm.Deconvolver(600, 500, list(weights), [k.copy() for k in kernels]) as dec:
+            first = dec.autotune(planes=4)
+        lines = path.read_text().splitlines()
+        assert len(lines) == 1 and lines[0].split()[0].endswith("|p4")
+        with fm.Deconvolver(600, 500, list(weights), [k.copy() for k in kernels]) as dec:
+            second = dec.autotune(planes=4)
+        assert {k: c for k, (c, _) in first.items()} == {k: c for k, (c, _) in second.items()}
+        assert path.read_text().splitlines() == lines
+    finally:
+        fm.lib().fmvd_set_autotune_cache(None)
