@@ -83,6 +83,8 @@ def lib() -> C.CDLL:
         "fmvd_deconvolve_volume_device_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_deconvolve_volume_device_noalloc_cuda_16": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
         "fmvd_deconvolve_volume_device_noalloc_cuda_8": (i, [vp, C.POINTER(vp), i, i, vp, vp, vp, vp]),
+        "fmvd_set_autotune_cache": (None, [cp]),
+        "fmvd_autotune_cache_selftest": (i, []),
         "fmvd_set_lookahead_cuda_16": (i, [vp, i]),
         "fmvd_set_lookahead_cuda_8": (i, [vp, i]),
         "fmvd_set_weights_device_cuda_16": (i, [vp, C.POINTER(vp), vp]),

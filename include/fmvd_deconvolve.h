@@ -204,6 +204,18 @@ int fmvd_deconvolve_volume_device_noalloc_cuda_8(const struct fmvd_plan_cuda_8 *
  * safe to call while the plan is running. */
 int fmvd_autotune_cuda_16(struct fmvd_plan_cuda_16 *plan, int planes, int *chosen, float *times_ms);
 int fmvd_autotune_cuda_8(struct fmvd_plan_cuda_8 *plan, int planes, int *chosen, float *times_ms);
+/* Autotuner cache file (no counterpart in the reference).  With a path set,
+ * fmvd_autotune_cuda_* first looks the plan's configuration up in that text
+ * file (device, FFT extents, sample type, kernel-hat form, planes per launch)
+ * and applies the recorded picks without timing; otherwise it times every
+ * variant and appends a line with the picks and times.  NULL or "" turns the
+ * cache off; until this is called, the environment variable
+ * FMVD_AUTOTUNE_CACHE sets the path.  fmvd_autotune_cache_selftest checks the
+ * line format (a round trip, and rejection of malformed lines) without a
+ * device and returns 0 when it holds. */
+void fmvd_set_autotune_cache(const char *path);
+int fmvd_autotune_cache_selftest(void);
+
 /* Sets the variants directly (variants[5] as above; the egress kernel has no
  * FFT plan, so variants[4] must be 0).  Returns -1 if a value is out of range
  * for this plan's extents. */
