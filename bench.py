@@ -65,6 +65,9 @@ def parse():
     p.add_argument("--no-learned", action="store_true", help="skip the learned warm-start legs")
     p.add_argument("--train-samples", type=int, default=2000)
     p.add_argument("--epochs", type=int, default=3)
+    p.add_argument("--learned-study", action="store_true",
+                   help="only the warm-start convergence study: networks trained to a loss plateau on labels "
+                        "of 20 and 50 plain-RL iterations, trajectories against the FP64 50-iteration judge")
     p.add_argument("--traffic-run", action="store_true",
                    help="short run for HBM counter collection: 64 planes, 2 iterations, no other legs, then "
                         "a 1 GiB device-to-device copy as the counters' calibration kernel")
@@ -323,7 +326,9 @@ def learned_leg(fm, oracle, ntrain=2000, epochs=3, label_iters=50, ntest=2, lr=2
         k_equiv = next((k for k, e in enumerate(ep) if e <= best), None)
         accepted = kw is not None and plain_s is not None and infer_s + warm_iter_s < plain_s
         learned_s = infer_s + warm_iter_s if accepted else infer_s + (plain_s or 0.0)
+        saved, saved_j, saved_e = iterations_saved(ep, ew)
         results.append(dict(k_plain=kp, k_warm=kw, accepted=accepted, plain_s=plain_s, learned_s=learned_s,
+                            max_iterations_saved=saved, saved_at_warm_k=saved_j, saved_at_error=saved_e,
                             infer_s=infer_s, speedup=(plain_s / learned_s) if plain_s and learned_s else None,
                             net_rel_l2=float(ew[0]), best_warm_rel_l2=best, best_warm_k=int(ew.argmin()),
                             plain_k_to_best=k_equiv, oracle_s=oracle_s,
@@ -342,6 +347,77 @@ def learned_leg(fm, oracle, ntrain=2000, epochs=3, label_iters=50, ntest=2, lr=2
     return leg, net
 
 
+def iterations_saved(plain, warm):
+    """For every point j of the warm trajectory: the plain iterations needed
+    to reach the same error, minus j.  Returns the best (saved, j, error)."""
+    best = (0, 0, float(warm[0]))
+    for j, e in enumerate(warm):
+        k = next((k for k, p in enumerate(plain) if p <= e), None)
+        if k is not None and k - j > best[0]:
+            best = (k - j, j, float(e))
+    return best
+
+
+def learned_study(fm, oracle, ntrain=2000, epochs=10, label_sets=(20, 50), judge_iters=50, ntest=4, seed=11):
+    """Convergence study of the warm start: for each label depth k0, a network
+    trained until its loss stops falling on (views -> k0-iteration plain RL)
+    pairs; on held-out seeds, the relative-L2 distance to the FP64 oracle's
+    judge_iters-iteration RL after every iteration of plain RL and of RL
+    started from the network."""
+    from gpu_deconvolution_amd import warmstart as ws
+    shape, V = (64, 128, 128), 2
+    Z, H, W = shape
+    nt, _ = cpu_threads()
+    out = {"workload": f"{ntrain} training samples 128x128x64 x 2 views, batch 8 Adam, up to {epochs} epochs "
+                       f"(stop when the epoch loss falls by less than 2%); {ntest} held-out seeds; judge: FP64 "
+                       f"oracle {judge_iters}-iteration cold-start RL", "nets": []}
+    test = ws.make_samples(ntest, shape, V, 31, judge_iters, seed=seed + 999, log=log)
+    refs = []
+    for s in test:
+        data = np.stack([v.download() for v in s.views], 1)
+        _, ref = oracle.rl_planes(data, np.stack(s.weights), np.stack(s.kernels).copy(), 0, judge_iters, threads=nt)
+        refs.append(fm.DeviceArray.from_host(ref.astype(np.float32)))
+    plain = []
+    for s, dref in zip(test, refs):
+        with fm.Deconvolver(H, W, s.weights, [k.copy() for k in s.kernels]) as dec:
+            plain.append(dec.convergence([v.ptr for v in s.views], Z, judge_iters + 20, dref.ptr))
+    for k0 in label_sets:
+        samples = ws.make_samples(ntrain, shape, V, 31, k0, seed=seed, log=log)
+        net = fm.Net(seed)
+        tr = ws.Trainer(net, shape, V, batch=8)
+        hist, lr = [], 4e-4
+        for ep in range(epochs):
+            hist += tr.fit(samples, 1, lr=lr, seed=ep, log=log)
+            lr *= 0.8
+            if len(hist) >= 3 and hist[-1] > 0.98 * hist[-2]:
+                break
+        tr.free()
+        for s in samples:
+            s.free()
+        rows = []
+        init = fm.DeviceArray(shape, np.float32)
+        for s, dref, ep in zip(test, refs, plain):
+            ws.warm_start(net, s.views, s.scale, init)
+            with fm.Deconvolver(H, W, s.weights, [k.copy() for k in s.kernels]) as dec:
+                ew = dec.convergence([v.ptr for v in s.views], Z, judge_iters + 20, dref.ptr, init_ptr=init.ptr)
+            saved, j, e = iterations_saved(ep, ew)
+            rows.append(dict(net_rel_l2=float(ew[0]), best_warm_rel_l2=float(ew.min()), best_warm_k=int(ew.argmin()),
+                             k_warm=first_below(ew), k_plain=first_below(ep), max_saved=saved, saved_at_k=j,
+                             saved_at_error=e, warm_errs=[float(x) for x in ew[:41:2]],
+                             plain_errs=[float(x) for x in ep[:41:2]]))
+            log(f"[study] k0={k0}: net {ew[0]:.3e}, best warm {ew.min():.3e}@{int(ew.argmin())}, saves {saved} "
+                f"iterations at error {e:.3e} (warm iteration {j}); plain reaches 1e-3 at {first_below(ep)}, "
+                f"warm at {first_below(ew)}")
+        init.free()
+        net.close()
+        out["nets"].append(dict(label_iterations=k0, loss_per_epoch=hist, samples=rows))
+    for r in refs:
+        r.free()
+    for s in test:
+        s.free()
+    return out
+
+
 # ------------------------------------------------------------------ main
 def main():
     args = parse()
@@ -351,6 +427,12 @@ def main():
         sys.exit(2)
     import gpu_deconvolution_amd as fm
     from gpu_deconvolution_amd import synth
+
+    if args.learned_study:
+        import oracle
+        fm.lib().setCudaDevice(0)
+        print(json.dumps({"learned_study": learned_study(fm, oracle)}), flush=True)
+        return
 
     Z, H, W, V, K = args.planes, 2048, 2048, 4, 41
     shape = (Z, H, W)
