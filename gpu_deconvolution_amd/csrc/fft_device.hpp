@@ -308,7 +308,8 @@ template <int... Rs> struct RadixList {
     }
 };
 
-// G_ (optional): rows per workgroup of the row kernels (0 = their default).
+// G_ (optional): rows per workgroup of the row kernels, sequences per
+// workgroup of the column kernel (0 = their default).
 // W_ (optional): minimum resident waves per SIMD the kernels are compiled for
 // (amdgpu_waves_per_eu; 1 = whatever the workgroup size implies); a higher
 // value caps the VGPRs per thread so that more workgroups fit on a CU.
@@ -382,7 +383,9 @@ FMVD_PLAN_VARIANT_GW(1280, 1, 64, 4, 4, 4, 4, 4, 20)
 FMVD_PLAN_VARIANT(1280, 2, 64, 4, 4, 4, 4, 5)
 FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
 FMVD_PLAN_VARIANT(1536, 2, 128, 4, 4, 4, 4, 6)
-FMVD_PLAN_VARIANT(2560, 1, 64, 4, 8, 8, 10)
+// one 2-wavefront sequence per column workgroup: 22 KB of LDS, 7 workgroups
+// per CU where two sequences (44 KB) allow 3
+FMVD_PLAN_VARIANT_GW(2560, 1, 128, 1, 4, 4, 4, 4, 4, 10)
 FMVD_PLAN_VARIANT(2560, 2, 128, 4, 4, 4, 4, 10)
 #undef FMVD_PLAN_VARIANT
 #undef FMVD_PLAN_VARIANT_G

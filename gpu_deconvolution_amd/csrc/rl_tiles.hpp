@@ -29,7 +29,7 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblocks)
 // consecutive columns, which then land on 16 distinct bank pairs.
 constexpr int row_pitch(int len, int G)
 {
-    const int want = G >= 16 ? 1 : 16 / G;
+    const int want = G >= 16 ? 1 : (16 / G) % 16;  // one row (G = 1): any multiple of 16
     int L = len;
     while (L % 16 != want) ++L;
     return L;
@@ -423,7 +423,7 @@ template <typename PL_> struct ColCfg {
     static constexpr int SQ0 = (256 / T) < 1 ? 1 : (256 / T);
     static constexpr int SQ1 = SQ0 > 16 ? 16 : SQ0;
     static constexpr int SQ2 = 65536 / (L * 8) < 1 ? 1 : 65536 / (L * 8);
-    static constexpr int SQ = pow2_floor(SQ1 < SQ2 ? SQ1 : SQ2);
+    static constexpr int SQ = PL::G ? PL::G : pow2_floor(SQ1 < SQ2 ? SQ1 : SQ2);
     static constexpr int THREADS = SQ * T;
     static constexpr int RF = first_radix<PL, -1>();
     static constexpr int RL = last_radix<PL, -1>();
