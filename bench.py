@@ -140,7 +140,7 @@ def kernel_stats(timing, fh, fw, planes_per_launch):
     return ks, B_RL, iter_gbs
 
 
-def cpu_baseline(oracle, registered, weights, kernels, shape, planes=2, iters=1):
+def cpu_baseline(oracle, registered, weights, kernels, shape, planes=4, iters=3):
     """FP64 oracle and its FP32 twin on `planes` planes x all views x `iters`
     iterations of the registered volume; plan setup (a 0-iteration run) is
     subtracted."""

@@ -581,6 +581,50 @@ FMVD_HD void pass_twiddle(float2* v, int t, const float2* __restrict__ tw)
     }
 }
 
+// The same twiddles split into a load (into w, NB (R-1) values in the order
+// pass_twiddle uses them) and an apply, so that a pass can issue its twiddle
+// loads before the LDS exchange that precedes it and have them arrive during
+// the exchange.
+template <typename PL, int I> constexpr int tw_regs()
+{
+    return span_before<PL>(I) > 1 ? (PL::P / PL::R::at(I)) * (PL::R::at(I) - 1) : 1;
+}
+template <typename PL, int I>
+FMVD_HD void load_twiddles(float2* w, int t, const float2* __restrict__ tw)
+{
+    constexpr int R = PL::R::at(I), NB = PL::P / R, NS = span_before<PL>(I);
+    if constexpr (NS > 1) {
+        const float2* twp = tw + tw_pass_offset<PL>(I);
+        if constexpr (affine_tw<PL, I>()) {
+            const float2* tb = twp + tw_k<PL, I>(t, 0);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = tb[tw_k<PL, I>(0, b) + (r - 1) * NS];
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float2* tb = twp + tw_k<PL, I>(t, b);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = tb[(r - 1) * NS];
+            }
+        }
+    }
+}
+template <typename PL, int I, bool CONJ> FMVD_HD void apply_twiddles(float2* v, const float2* w)
+{
+    constexpr int R = PL::R::at(I), NB = PL::P / R, NS = span_before<PL>(I);
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const float2 x = w[b * (R - 1) + r - 1];
+                v[b * R + r] = CONJ ? cmulc(v[b * R + r], x) : cmul(v[b * R + r], x);
+            }
+    }
+}
+
 template <typename PL, int I, int DIR> FMVD_HD void pass_dft(float2* v)
 {
     constexpr int R = PL::R::at(I), NB = PL::P / R;
@@ -612,12 +656,17 @@ __device__ __forceinline__ void fwd_passes(float2* v, float2* lds, int t, const 
 {
     if constexpr (I < PL::NP) {
         if constexpr (I > 0) {
+            // this pass's twiddles load while the exchange runs
+            float2 w[tw_regs<PL, I>()];
+            load_twiddles<PL, I>(w, t, tw);
             lds_out_slots<PL, I - 1, true>(lds, v, t);
             seq_sync<PL>();
             lds_in_slots<PL, I, false>(lds, v, t);
             seq_sync<PL>();
+            apply_twiddles<PL, I, false>(v, w);
+        } else {
+            pass_twiddle<PL, I, false>(v, t, tw);
         }
-        pass_twiddle<PL, I, false>(v, t, tw);
         pass_dft<PL, I, -1>(v);
         fwd_passes<PL, I + 1>(v, lds, t, tw);
     }

@@ -42,14 +42,17 @@ template <typename PL> struct Emu {
     {
         if constexpr (I < PL::NP) {
             if constexpr (I > 0) {
+                // as fwd_passes: the twiddles are loaded before the exchange
+                std::vector<float2> w((size_t)T * tw_regs<PL, I>());
+                for (int t = 0; t < T; ++t) load_twiddles<PL, I>(w.data() + (size_t)t * tw_regs<PL, I>(), t, tw.data());
                 reset_lds();
                 for (int t = 0; t < T; ++t) lds_out_slots<PL, I - 1, true>(L(), V(t), t);
                 for (int t = 0; t < T; ++t) lds_in_slots<PL, I, false>(L(), V(t), t);
+                for (int t = 0; t < T; ++t) apply_twiddles<PL, I, false>(V(t), w.data() + (size_t)t * tw_regs<PL, I>());
+            } else {
+                for (int t = 0; t < T; ++t) pass_twiddle<PL, I, false>(V(t), t, tw.data());
             }
-            for (int t = 0; t < T; ++t) {
-                pass_twiddle<PL, I, false>(V(t), t, tw.data());
-                pass_dft<PL, I, -1>(V(t));
-            }
+            for (int t = 0; t < T; ++t) pass_dft<PL, I, -1>(V(t));
             fwd<I + 1>();
         }
     }
