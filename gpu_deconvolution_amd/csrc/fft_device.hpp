@@ -677,14 +677,27 @@ template <typename PL, int I>
 __device__ __forceinline__ void inv_passes(float2* v, float2* lds, int t, const float2* __restrict__ tw)
 {
     if constexpr (I >= 0) {
-        if constexpr (I < PL::NP - 1) {
+        if constexpr (I < PL::NP - 1 && PL::R::at(I) <= 4) {
+            // small-radix passes: the twiddles load while the exchange runs
+            // (they are applied after the DFT, whose few temporaries leave room)
+            float2 w[tw_regs<PL, I>()];
+            load_twiddles<PL, I>(w, t, tw);
             lds_in_slots<PL, I + 1, true>(lds, v, t);
             seq_sync<PL>();
             lds_out_slots<PL, I, false>(lds, v, t);
             seq_sync<PL>();
+            pass_dft<PL, I, +1>(v);
+            apply_twiddles<PL, I, true>(v, w);
+        } else {
+            if constexpr (I < PL::NP - 1) {
+                lds_in_slots<PL, I + 1, true>(lds, v, t);
+                seq_sync<PL>();
+                lds_out_slots<PL, I, false>(lds, v, t);
+                seq_sync<PL>();
+            }
+            pass_dft<PL, I, +1>(v);
+            pass_twiddle<PL, I, true>(v, t, tw);
         }
-        pass_dft<PL, I, +1>(v);
-        pass_twiddle<PL, I, true>(v, t, tw);
         inv_passes<PL, I - 1>(v, lds, t, tw);
     }
 }

@@ -59,6 +59,11 @@ template <typename PL> struct Emu {
     template <int I> void inv()
     {
         if constexpr (I >= 0) {
+            // as inv_passes: small-radix passes load their twiddles before the exchange
+            constexpr bool pre = I < PL::NP - 1 && PL::R::at(I) <= 4;
+            std::vector<float2> w((size_t)T * tw_regs<PL, I>());
+            if constexpr (pre)
+                for (int t = 0; t < T; ++t) load_twiddles<PL, I>(w.data() + (size_t)t * tw_regs<PL, I>(), t, tw.data());
             if constexpr (I < PL::NP - 1) {
                 reset_lds();
                 for (int t = 0; t < T; ++t) lds_in_slots<PL, I + 1, true>(L(), V(t), t);
@@ -66,7 +71,8 @@ template <typename PL> struct Emu {
             }
             for (int t = 0; t < T; ++t) {
                 pass_dft<PL, I, +1>(V(t));
-                pass_twiddle<PL, I, true>(V(t), t, tw.data());
+                if constexpr (pre) apply_twiddles<PL, I, true>(V(t), w.data() + (size_t)t * tw_regs<PL, I>());
+                else pass_twiddle<PL, I, true>(V(t), t, tw.data());
             }
             inv<I - 1>();
         }
