@@ -232,7 +232,7 @@ def batched_workload(fm, synth, seed=2024, nvol=48, iters=25, kernel=31):
     xy 1-2.5 / z 3-6, Poisson uint16), 25 iterations replayed as a hipGraph,
     FFT plans chosen by the batch autotuner."""
     rng = np.random.default_rng(seed)
-    vols, vvi = [], 0
+    vols, vvi, brl = [], 0, 0
     t0 = time.time()
     for i in range(nvol):
         xy = int(rng.choice([128, 192, 256, 384, 512]))
@@ -245,6 +245,8 @@ def batched_workload(fm, synth, seed=2024, nvol=48, iters=25, kernel=31):
         vols.append(dict(views=views, weights=[np.ones((xy, xy), np.float32) for _ in range(V)], kernels=kernels,
                          out=fm.DeviceArray((z, xy, xy), np.uint16)))
         vvi += V * z * xy * xy
+        fh = fw = fm.snap_transform_size(xy + kernel - 1)
+        brl += V * z * (64 * fh * (fw // 2 + 1) + 10 * fh * fw)  # B_RL of every (view, plane), one iteration
     log(f"[bench] batched workload: {nvol} volumes generated in {time.time() - t0:.1f}s")
     b = fm.Batch(vols)
     classes = b.info()
@@ -255,7 +257,8 @@ def batched_workload(fm, synth, seed=2024, nvol=48, iters=25, kernel=31):
     b.run(iters, use_graph=True)
     fm.device_synchronize()
     el = time.perf_counter() - t
-    log(f"[bench] batched: {iters} iterations in {el:.3f}s -> {vvi * iters / el:.4g} view-voxel it/s; "
+    log(f"[bench] batched: {iters} iterations in {el:.3f}s -> {vvi * iters / el:.4g} view-voxel it/s, "
+        f"{brl * iters / el / 1e9:.0f} GB/s algorithmic; "
         f"classes {classes}, variants {tuned}")
     b.close()
     for v in vols:
@@ -265,7 +268,11 @@ def batched_workload(fm, synth, seed=2024, nvol=48, iters=25, kernel=31):
     return {"workload": f"{nvol} independent reconstructions, xy in {{128,192,256,384,512}}, z 32-256, 2-4 views, "
                         f"{kernel}x{kernel} PSFs, {iters} iterations as hipGraph replays",
             "value": vvi * iters / el, "unit": "view-voxel RL iterations/s", "seconds": el,
-            "classes": classes, "fft_plan_variants": tuned}
+            "classes": classes, "fft_plan_variants": tuned,
+            "rl_iteration_roofline": {"achieved": brl * iters / el / 1e9, "frac": brl * iters / el / 1e9 / HBM_PEAK_GBS,
+                                      "unit": "GB/s", "note": "B_RL = 64 S + 10 P per view, iteration and plane of "
+                                      "each volume's FFT class; the 256^2 and 512^2 classes are cache-resident, "
+                                      "so the figure can exceed what HBM alone allows"}}
 
 
 def learned_leg(fm, oracle, ntrain=2000, epochs=3, label_iters=50, ntest=2, lr=2e-4, seed=7, tol=TOL):
