@@ -60,3 +60,16 @@ def test_noalloc_small_planes(fm):
         for Z in (40, 7):
             (oa, ea), (ob, eb) = run_both(fm, dec, volume(rng, Z, V, H, W, 16), Z, 4)
             assert np.array_equal(oa, ob) and np.array_equal(ea, eb)
+
+
+def test_noalloc_rejects_planes_beyond_its_scratch(fm):
+    """A plane whose spectrum and estimate exceed the fixed scratch (FFT
+    8192 x 2560: 168 MB) is rejected with an error, before any launch."""
+    H, W = 8150, 2500
+    kernels = [np.ones((41, 41), np.float32) for _ in range(2)]
+    with fm.Deconvolver(H, W, None, kernels) as dec:
+        assert (dec.info()["fft_h"], dec.info()["fft_w"]) == (8192, 2560)
+        d = fm.DeviceArray((1, H, W), np.uint16)
+        with pytest.raises(fm.FmvdError, match="allocation-free"):
+            dec.run_device_noalloc([d.ptr, d.ptr], 1, 1)
+        d.free()
