@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--steps", type=int, default=30, help="RL iterations in the timed reconstruction")
     p.add_argument("--warmup", type=int, default=2, help="RL iterations of the untimed reconstruction")
     p.add_argument("--planes", type=int, default=1024, help="volume depth (default: the workload's 1024)")
+    p.add_argument("--chunk-planes", type=int, default=0,
+                   help="planes per launch of the headline plan (0 = the plan's automatic choice)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-judge", action="store_true", help="skip the time-to-1e-3 legs")
     p.add_argument("--no-streamed", action="store_true", help="skip the 1024^2 x 512 streamed leg")
@@ -454,8 +456,10 @@ def main():
     out = fm.DeviceArray(shape, np.uint16)
     log(f"[bench] data ready in {time.time() - t0:.1f}s on {name.value.decode()}")
 
-    dec = fm.Deconvolver(H, W, None, [k.copy() for k in kernels], psf_type=0, bits=16, batch_planes=min(Z, 256))
+    dec = fm.Deconvolver(H, W, None, [k.copy() for k in kernels], psf_type=0, bits=16, batch_planes=min(Z, 256),
+                         chunk_planes=args.chunk_planes)
     info = dec.info()
+    lanes = int(os.environ.get("FMVD_LANES", "1") or 1)
     stream = fm.Stream()
 
     def reconstruct(iters):
@@ -482,7 +486,7 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = V * W * H * Z * args.steps / elapsed
     log(f"[bench] {args.steps} iterations (+4 transforms, mask weights) in {elapsed:.3f}s -> {ms_per_step:.1f} "
-        f"ms/step, {value:.4g} view-voxel it/s")
+        f"ms/step, {value:.4g} view-voxel it/s (chunk {info['chunk']}, lanes {lanes})")
 
     fh, fw = info["fft_h"], info["fft_w"]
     planes_per_launch = min(Z, info["batch"], info["chunk"])

@@ -35,6 +35,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 // Host-and-device inline: the pass code also runs in the host emulator that
@@ -112,6 +113,24 @@ template <int DIR>
 FMVD_HD constexpr float2 ct_twiddle(long m, long n)
 {
     return make_float2(float(ct_cos2pi(m, n)), float(DIR * ct_sin2pi(m, n)));
+}
+
+// exp(DIR 2 pi i M / N) as a compile-time constant
+template <int DIR, long M, long N> FMVD_HD float2 ct_tw()
+{
+    constexpr float x = float(ct_cos2pi(M, N)), y = float(DIR * ct_sin2pi(M, N));
+    return make_float2(x, y);
+}
+
+// Compile-time loop: f.template operator()<i>() for i = B .. E-1, so that
+// constants derived from i are evaluated by the front end, not left to the
+// optimiser after unrolling.
+template <int B, int E, typename F> FMVD_HD void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f.template operator()<B>();
+        static_for<B + 1, E>(f);
+    }
 }
 
 // ------------------------------------------------------ in-register DFTs
@@ -313,12 +332,18 @@ template <int... Rs> struct RadixList {
 // W_ (optional): minimum resident waves per SIMD the kernels are compiled for
 // (amdgpu_waves_per_eu; 1 = whatever the workgroup size implies); a higher
 // value caps the VGPRs per thread so that more workgroups fit on a CU.
-template <int N_, int T_, typename Radices, int G_ = 0, int W_ = 1> struct PlanDef {
+// S_ (optional): split exchanges.  Every LDS exchange moves the real parts,
+// then the imaginary parts, through a float buffer (E = float) instead of
+// moving complex values through a float2 buffer (E = float2): half the LDS per
+// sequence, for twice the synchronisations.
+template <int N_, int T_, typename Radices, int G_ = 0, int W_ = 1, bool S_ = false> struct PlanDef {
     static constexpr int N = N_;
     static constexpr int T = T_;
     static constexpr int G = G_;
     static constexpr int WPE = W_;
     static constexpr int P = N_ / T_;
+    static constexpr bool SPLIT = S_;
+    using E = std::conditional_t<S_, float, float2>;
     using R = Radices;
     static constexpr int NP = Radices::count;
 };
@@ -366,11 +391,12 @@ template <int N, int V> struct PlanV : Plan<N> {
     static constexpr bool distinct = (V == 0);
     static constexpr int variant = 0;
 };
-#define FMVD_PLAN_VARIANT_GW(n, v, t, g, w, ...)                                   \
-    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>, g, w> {  \
-        static constexpr bool distinct = true;                                     \
-        static constexpr int variant = v;                                          \
+#define FMVD_PLAN_VARIANT_GWS(n, v, t, g, w, s, ...)                                  \
+    template <> struct PlanV<n, v> : PlanDef<n, t, RadixList<__VA_ARGS__>, g, w, s> {  \
+        static constexpr bool distinct = true;                                        \
+        static constexpr int variant = v;                                             \
     };
+#define FMVD_PLAN_VARIANT_GW(n, v, t, g, w, ...) FMVD_PLAN_VARIANT_GWS(n, v, t, g, w, false, __VA_ARGS__)
 #define FMVD_PLAN_VARIANT_G(n, v, t, g, ...) FMVD_PLAN_VARIANT_GW(n, v, t, g, 1, __VA_ARGS__)
 #define FMVD_PLAN_VARIANT(n, v, t, ...) FMVD_PLAN_VARIANT_G(n, v, t, 0, __VA_ARGS__)
 FMVD_PLAN_VARIANT(256, 1, 32, 8, 8, 4)
@@ -380,16 +406,18 @@ FMVD_PLAN_VARIANT_GW(768, 2, 64, 4, 6, 4, 4, 4, 12)
 FMVD_PLAN_VARIANT(1024, 1, 64, 8, 8, 16)
 FMVD_PLAN_VARIANT(1024, 2, 128, 8, 8, 4, 4)
 FMVD_PLAN_VARIANT_GW(1280, 1, 64, 4, 4, 4, 4, 4, 20)
-FMVD_PLAN_VARIANT(1280, 2, 64, 4, 4, 4, 4, 5)
+// variant 1 with split exchanges: 22 KB of LDS per 4-row workgroup, so four
+// workgroups (4 waves per SIMD) fit where variant 1's 44 KB allow three
+FMVD_PLAN_VARIANT_GWS(1280, 2, 64, 4, 4, true, 4, 4, 4, 20)
 FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
 FMVD_PLAN_VARIANT(1536, 2, 128, 4, 4, 4, 4, 6)
-// one 2-wavefront sequence per column workgroup: 22 KB of LDS, 7 workgroups
-// per CU where two sequences (44 KB) allow 3
-FMVD_PLAN_VARIANT_GW(2560, 1, 128, 1, 4, 4, 4, 4, 4, 10)
+// variant 2 with split exchanges: two 2-wavefront sequences in 22 KB of LDS
+FMVD_PLAN_VARIANT_GWS(2560, 1, 128, 2, 4, true, 4, 4, 4, 4, 10)
 FMVD_PLAN_VARIANT(2560, 2, 128, 4, 4, 4, 4, 10)
 #undef FMVD_PLAN_VARIANT
 #undef FMVD_PLAN_VARIANT_G
 #undef FMVD_PLAN_VARIANT_GW
+#undef FMVD_PLAN_VARIANT_GWS
 
 // A plan is valid when its radices multiply to N and each divides P.
 template <typename PL> constexpr bool plan_valid()
@@ -496,111 +524,119 @@ template <typename PL, int I> constexpr bool affine_tw()
     return true;
 }
 
+// Component COMP of a complex value as one LDS element: 0 the complex value
+// (float2), 1 its real part, 2 its imaginary part (float).
+template <int COMP> struct Comp;
+template <> struct Comp<0> {
+    using E = float2;
+    static FMVD_HD float2& of(float2& v) { return v; }
+};
+template <> struct Comp<1> {
+    using E = float;
+    static FMVD_HD float& of(float2& v) { return v.x; }
+};
+template <> struct Comp<2> {
+    using E = float;
+    static FMVD_HD float& of(float2& v) { return v.y; }
+};
+
 // LDS traffic of pass I.  "in" slots are the positions the forward reads
 // (and the inverse writes), "out" slots those the forward writes (and the
-// inverse reads).  v[b*R + r] is slot (b, r).
-template <typename PL, int I, bool STORE>
-FMVD_HD void lds_in_slots(float2* lds, float2* v, int t)
+// inverse reads).  v[b*R + r] is slot (b, r).  COMP selects what moves: the
+// complex values through a float2 buffer, or one part through a float buffer.
+template <typename PL, int I, bool STORE, int COMP = 0>
+FMVD_HD void lds_in_slots(typename Comp<COMP>::E* lds, float2* v, int t)
 {
+    using X = Comp<COMP>;
     constexpr int R = PL::R::at(I), NB = PL::P / R;
     if constexpr (affine_in<PL, I>()) {
-        float2* base = lds + lds_pad(t);
+        typename X::E* base = lds + lds_pad(t);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float2& x = base[lds_pad(in_pos_p<PL, I>(0, b, r))];
-                if constexpr (STORE) x = v[b * R + r];
-                else v[b * R + r] = x;
+                typename X::E& x = base[lds_pad(in_pos_p<PL, I>(0, b, r))];
+                if constexpr (STORE) x = X::of(v[b * R + r]);
+                else X::of(v[b * R + r]) = x;
             }
     } else {
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float2& x = lds[lds_pad(in_pos_p<PL, I>(t, b, r))];
-                if constexpr (STORE) x = v[b * R + r];
-                else v[b * R + r] = x;
+                typename X::E& x = lds[lds_pad(in_pos_p<PL, I>(t, b, r))];
+                if constexpr (STORE) x = X::of(v[b * R + r]);
+                else X::of(v[b * R + r]) = x;
             }
     }
 }
-template <typename PL, int I, bool STORE>
-FMVD_HD void lds_out_slots(float2* lds, float2* v, int t)
+template <typename PL, int I, bool STORE, int COMP = 0>
+FMVD_HD void lds_out_slots(typename Comp<COMP>::E* lds, float2* v, int t)
 {
+    using X = Comp<COMP>;
     constexpr int R = PL::R::at(I), NB = PL::P / R;
     if constexpr (affine_out<PL, I>()) {
-        float2* base = lds + lds_pad(out_pos_p<PL, I>(t, 0, 0));
+        typename X::E* base = lds + lds_pad(out_pos_p<PL, I>(t, 0, 0));
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float2& x = base[lds_pad(out_pos_p<PL, I>(0, b, r))];
-                if constexpr (STORE) x = v[b * R + r];
-                else v[b * R + r] = x;
+                typename X::E& x = base[lds_pad(out_pos_p<PL, I>(0, b, r))];
+                if constexpr (STORE) x = X::of(v[b * R + r]);
+                else X::of(v[b * R + r]) = x;
             }
     } else {
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float2& x = lds[lds_pad(out_pos_p<PL, I>(t, b, r))];
-                if constexpr (STORE) x = v[b * R + r];
-                else v[b * R + r] = x;
+                typename X::E& x = lds[lds_pad(out_pos_p<PL, I>(t, b, r))];
+                if constexpr (STORE) x = X::of(v[b * R + r]);
+                else X::of(v[b * R + r]) = x;
             }
     }
 }
 
 // Twiddles of pass I: forward multiplies input r by w^(k r), the inverse
-// multiplies output r of its inverse DFT by the conjugate.
-template <typename PL, int I, bool CONJ>
-FMVD_HD void pass_twiddle(float2* v, int t, const float2* __restrict__ tw)
+// multiplies output r of its inverse DFT by the conjugate;
+// w = exp(-2 pi i / (R NS)), k = tw_k(t, b).
+//
+// Where the twiddle addresses are affine (every plan's pass that has
+// twiddles), they are generated rather than loaded one by one: each thread
+// loads w^k and, for radices above 8, w^(Q k) with Q = ceil(sqrt(R)) from the
+// pass's table, and forms w^(k r) = w^(k a) w^(Q k m) (r = a + Q m) with the
+// powers built by binary splitting, so at most five rounded products deep.
+// A butterfly b whose k is b = 0's plus a compile-time d_b further multiplies
+// by the constant w^(d_b r).  One or two loads per pass replace NB (R - 1):
+// the loads' data return, not the arithmetic, bounds the passes on gfx950.
+template <int R> constexpr int tw_gen_q()
 {
-    constexpr int R = PL::R::at(I), NB = PL::P / R, NS = span_before<PL>(I);
-    if constexpr (NS > 1) {
-        const float2* twp = tw + tw_pass_offset<PL>(I);
-        if constexpr (affine_tw<PL, I>()) {
-            const float2* tb = twp + tw_k<PL, I>(t, 0);
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    const float2 w = tb[tw_k<PL, I>(0, b) + (r - 1) * NS];
-                    v[b * R + r] = CONJ ? cmulc(v[b * R + r], w) : cmul(v[b * R + r], w);
-                }
-        } else {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const float2* tb = twp + tw_k<PL, I>(t, b);
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    const float2 w = tb[(r - 1) * NS];
-                    v[b * R + r] = CONJ ? cmulc(v[b * R + r], w) : cmul(v[b * R + r], w);
-                }
-            }
-        }
-    }
+    if (R <= 8) return R;
+    int q = 1;
+    while (q * q < R) ++q;
+    return q;
 }
+template <typename PL, int I> constexpr bool tw_generated() { return span_before<PL>(I) > 1 && affine_tw<PL, I>(); }
 
-// The same twiddles split into a load (into w, NB (R-1) values in the order
-// pass_twiddle uses them) and an apply, so that a pass can issue its twiddle
-// loads before the LDS exchange that precedes it and have them arrive during
-// the exchange.
+// Registers a pass's twiddles occupy between their load and their use.
 template <typename PL, int I> constexpr int tw_regs()
 {
-    return span_before<PL>(I) > 1 ? (PL::P / PL::R::at(I)) * (PL::R::at(I) - 1) : 1;
+    if constexpr (tw_generated<PL, I>()) return 2;
+    else return span_before<PL>(I) > 1 ? (PL::P / PL::R::at(I)) * (PL::R::at(I) - 1) : 1;
 }
+
+// Load a pass's twiddles (generated: their base powers) into w.
 template <typename PL, int I>
 FMVD_HD void load_twiddles(float2* w, int t, const float2* __restrict__ tw)
 {
     constexpr int R = PL::R::at(I), NB = PL::P / R, NS = span_before<PL>(I);
     if constexpr (NS > 1) {
         const float2* twp = tw + tw_pass_offset<PL>(I);
-        if constexpr (affine_tw<PL, I>()) {
+        if constexpr (tw_generated<PL, I>()) {
+            constexpr int Q = tw_gen_q<R>();
             const float2* tb = twp + tw_k<PL, I>(t, 0);
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = tb[tw_k<PL, I>(0, b) + (r - 1) * NS];
+            w[0] = tb[0];                                    // w^k (r = 1)
+            if constexpr (Q < R) w[1] = tb[(Q - 1) * NS];    // w^(Q k) (r = Q)
         } else {
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -611,17 +647,54 @@ FMVD_HD void load_twiddles(float2* w, int t, const float2* __restrict__ tw)
         }
     }
 }
+
+// Multiply by the twiddles load_twiddles brought into w (CONJ: their conjugates).
 template <typename PL, int I, bool CONJ> FMVD_HD void apply_twiddles(float2* v, const float2* w)
 {
     constexpr int R = PL::R::at(I), NB = PL::P / R, NS = span_before<PL>(I);
     if constexpr (NS > 1) {
+        if constexpr (tw_generated<PL, I>()) {
+            constexpr int Q = tw_gen_q<R>(), M = (R + Q - 1) / Q;
+            float2 pa[Q], pm[M > 1 ? M : 1];
+            pa[1] = w[0];
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+            for (int n = 2; n < Q; ++n) pa[n] = cmul(pa[n / 2], pa[n - n / 2]);
+            if constexpr (M > 1) {
+                pm[1] = w[1];
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const float2 x = w[b * (R - 1) + r - 1];
-                v[b * R + r] = CONJ ? cmulc(v[b * R + r], x) : cmul(v[b * R + r], x);
+                for (int n = 2; n < M; ++n) pm[n] = cmul(pm[n / 2], pm[n - n / 2]);
             }
+            static_for<0, NB>([&]<int b>() {
+                static_for<1, R>([&]<int r>() {
+                    constexpr int a = r % Q, m = r / Q, d = tw_k<PL, I>(0, b);
+                    float2 x;
+                    if constexpr (a == 0) x = pm[m];
+                    else if constexpr (m == 0) x = pa[a];
+                    else x = cmul(pa[a], pm[m]);
+                    if constexpr (d != 0) x = cmul(x, ct_tw<-1, (long)d * r, (long)R * NS>());
+                    v[b * R + r] = CONJ ? cmulc(v[b * R + r], x) : cmul(v[b * R + r], x);
+                });
+            });
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 x = w[b * (R - 1) + r - 1];
+                    v[b * R + r] = CONJ ? cmulc(v[b * R + r], x) : cmul(v[b * R + r], x);
+                }
+        }
+    }
+}
+
+// Load and apply at once.
+template <typename PL, int I, bool CONJ>
+FMVD_HD void pass_twiddle(float2* v, int t, const float2* __restrict__ tw)
+{
+    if constexpr (span_before<PL>(I) > 1) {
+        float2 w[tw_regs<PL, I>()];
+        load_twiddles<PL, I>(w, t, tw);
+        apply_twiddles<PL, I, CONJ>(v, w);
     }
 }
 
@@ -649,20 +722,62 @@ template <typename PL> __device__ __forceinline__ void seq_sync()
     }
 }
 
+// The exchange before forward pass I: pass I-1's outputs go to their "out"
+// slots, pass I's inputs come from their "in" slots; for split plans the real
+// parts, then the imaginary parts.  Each read is ordered after every write
+// of its round and before the next round's writes.
+template <typename PL, int I>
+__device__ __forceinline__ void fwd_exchange(typename PL::E* lds, float2* v, int t)
+{
+    if constexpr (PL::SPLIT) {
+        lds_out_slots<PL, I - 1, true, 1>(lds, v, t);
+        seq_sync<PL>();
+        lds_in_slots<PL, I, false, 1>(lds, v, t);
+        seq_sync<PL>();
+        lds_out_slots<PL, I - 1, true, 2>(lds, v, t);
+        seq_sync<PL>();
+        lds_in_slots<PL, I, false, 2>(lds, v, t);
+        seq_sync<PL>();
+    } else {
+        lds_out_slots<PL, I - 1, true>(lds, v, t);
+        seq_sync<PL>();
+        lds_in_slots<PL, I, false>(lds, v, t);
+        seq_sync<PL>();
+    }
+}
+// The exchange before inverse pass I (transposed: pass I+1's "in" slots to
+// pass I's "out" slots).
+template <typename PL, int I>
+__device__ __forceinline__ void inv_exchange(typename PL::E* lds, float2* v, int t)
+{
+    if constexpr (PL::SPLIT) {
+        lds_in_slots<PL, I + 1, true, 1>(lds, v, t);
+        seq_sync<PL>();
+        lds_out_slots<PL, I, false, 1>(lds, v, t);
+        seq_sync<PL>();
+        lds_in_slots<PL, I + 1, true, 2>(lds, v, t);
+        seq_sync<PL>();
+        lds_out_slots<PL, I, false, 2>(lds, v, t);
+        seq_sync<PL>();
+    } else {
+        lds_in_slots<PL, I + 1, true>(lds, v, t);
+        seq_sync<PL>();
+        lds_out_slots<PL, I, false>(lds, v, t);
+        seq_sync<PL>();
+    }
+}
+
 // Forward passes I..NP-1; on entry v holds pass I-1's outputs (or, for I = 0,
 // the inputs at the "j + r*N/R0" layout).
 template <typename PL, int I>
-__device__ __forceinline__ void fwd_passes(float2* v, float2* lds, int t, const float2* __restrict__ tw)
+__device__ __forceinline__ void fwd_passes(float2* v, typename PL::E* lds, int t, const float2* __restrict__ tw)
 {
     if constexpr (I < PL::NP) {
         if constexpr (I > 0) {
             // this pass's twiddles load while the exchange runs
             float2 w[tw_regs<PL, I>()];
             load_twiddles<PL, I>(w, t, tw);
-            lds_out_slots<PL, I - 1, true>(lds, v, t);
-            seq_sync<PL>();
-            lds_in_slots<PL, I, false>(lds, v, t);
-            seq_sync<PL>();
+            fwd_exchange<PL, I>(lds, v, t);
             apply_twiddles<PL, I, false>(v, w);
         } else {
             pass_twiddle<PL, I, false>(v, t, tw);
@@ -674,7 +789,7 @@ __device__ __forceinline__ void fwd_passes(float2* v, float2* lds, int t, const 
 // Inverse passes I..0 (transposed forward passes, run backwards); on entry v
 // holds the values at pass I's "out" slots.
 template <typename PL, int I>
-__device__ __forceinline__ void inv_passes(float2* v, float2* lds, int t, const float2* __restrict__ tw)
+__device__ __forceinline__ void inv_passes(float2* v, typename PL::E* lds, int t, const float2* __restrict__ tw)
 {
     if constexpr (I >= 0) {
         if constexpr (I < PL::NP - 1 && PL::R::at(I) <= 4) {
@@ -682,19 +797,11 @@ __device__ __forceinline__ void inv_passes(float2* v, float2* lds, int t, const 
             // (they are applied after the DFT, whose few temporaries leave room)
             float2 w[tw_regs<PL, I>()];
             load_twiddles<PL, I>(w, t, tw);
-            lds_in_slots<PL, I + 1, true>(lds, v, t);
-            seq_sync<PL>();
-            lds_out_slots<PL, I, false>(lds, v, t);
-            seq_sync<PL>();
+            inv_exchange<PL, I>(lds, v, t);
             pass_dft<PL, I, +1>(v);
             apply_twiddles<PL, I, true>(v, w);
         } else {
-            if constexpr (I < PL::NP - 1) {
-                lds_in_slots<PL, I + 1, true>(lds, v, t);
-                seq_sync<PL>();
-                lds_out_slots<PL, I, false>(lds, v, t);
-                seq_sync<PL>();
-            }
+            if constexpr (I < PL::NP - 1) inv_exchange<PL, I>(lds, v, t);
             pass_dft<PL, I, +1>(v);
             pass_twiddle<PL, I, true>(v, t, tw);
         }
@@ -709,7 +816,7 @@ __device__ __forceinline__ void inv_passes(float2* v, float2* lds, int t, const 
 // have ordered any earlier LDS reads of this sequence's slice before the
 // call (seq_sync or a workgroup barrier).
 template <typename PL, int DIR>
-__device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, const float2* __restrict__ tw)
+__device__ __forceinline__ void fft_regs(float2* v, typename PL::E* lds, int t, const float2* __restrict__ tw)
 {
     if constexpr (DIR < 0) fwd_passes<PL, 0>(v, lds, t, tw);
     else inv_passes<PL, PL::NP - 1>(v, lds, t, tw);

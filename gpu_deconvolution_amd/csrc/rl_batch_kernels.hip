@@ -47,7 +47,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void batch_ratio_kern
                                                                           float2* __restrict__ XT, long xstride)
 {
     using C = RowCfg<PL>;
-    __shared__ float2 lds[C::G * C::L];
+    __shared__ typename C::E lds[C::G * C::L];
     const int lid = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int e = find_entry(list, n, lid);
     const BatchSeg& S = segs[list[e]];
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void batch_update_ker
                                                                            long estride)
 {
     using C = RowCfg<PL>;
-    __shared__ float2 lds[C::G * C::L];
+    __shared__ typename C::E lds[C::G * C::L];
     const int lid = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int e = find_entry(list, n, lid);
     const BatchSeg& S = segs[list[e]];
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void batch_col_kernel
                                                                         float2* Xout, long xstride)
 {
     using C = ColCfg<PL>;
-    __shared__ float2 lds[C::SQ * C::L];
+    __shared__ typename C::E lds[C::SQ * C::L];
     const int lid = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int e = find_entry(list, n, lid);
     const BatchSeg& S = segs[list[e]];

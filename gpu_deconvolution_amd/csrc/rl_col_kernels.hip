@@ -30,7 +30,7 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(
                                                                        float ksign)
 {
     using C = ColCfg<PL>;
-    __shared__ float2 lds[C::SQ * C::L];
+    __shared__ typename C::E lds[C::SQ * C::L];
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     // plane-fastest: consecutive logical blocks reuse the same kernel rows
     const int p = lid % nplanes, grp = lid / nplanes;

@@ -28,7 +28,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_ratio_kernel
     float scale)
 {
     using C = RowCfg<PL>;
-    __shared__ float2 lds[C::G * C::L];
+    __shared__ typename C::E lds[C::G * C::L];
     const int nrg = g.H / C::G;
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     const int rg = lid % nrg, p = lid / nrg;
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_update_kerne
     long estride, const float* __restrict__ weights, int nplanes)
 {
     using C = RowCfg<PL>;
-    __shared__ float2 lds[C::G * C::L];
+    __shared__ typename C::E lds[C::G * C::L];
     const int nrg = g.H / C::G;
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     int rg, p;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(RowCfg<PL>::THREADS, PL::WPE) void row_init_kernel(
     const float* __restrict__ e0, long e0stride, int nplanes)
 {
     using C = RowCfg<PL>;
-    __shared__ float2 lds[C::G * C::L];
+    __shared__ typename C::E lds[C::G * C::L];
     const int nrg = g.H / C::G;
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
     const int rg = lid % nrg, p = lid / nrg;

@@ -9,6 +9,16 @@
 
 namespace fmvd {
 
+// An opaque copy of a per-lane value.  Addresses derived from it cannot be
+// shared with the same addresses computed earlier (the tile's loads), so the
+// compiler recomputes them at the stores (two instructions each) instead of
+// holding one register per element for the whole kernel.
+__device__ __forceinline__ int fresh(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 __device__ __forceinline__ int pad_index(int i, int n, int half)
 {
     return i < n ? i : (i < n + half ? n - 1 : 0);
@@ -34,6 +44,16 @@ constexpr int row_pitch(int len, int G)
     while (L % 16 != want) ++L;
     return L;
 }
+// The same for the float buffer of split exchanges: 4-byte accesses are served
+// 32 lanes at a time, which then hold G rows times 32/G columns, so the pitch
+// (in floats) is 32/G (mod 32).
+constexpr int row_pitch_split(int len, int G)
+{
+    const int want = G >= 32 ? 1 : (32 / G) % 32;
+    int L = len;
+    while (L % 32 != want) ++L;
+    return L;
+}
 
 template <typename PL_> struct RowCfg {
     using PL = PL_;
@@ -46,13 +66,20 @@ template <typename PL_> struct RowCfg {
     // halved while the tile would exceed the 160 KiB of LDS a workgroup may
     // hold or 1024 threads (long rows: 4 rows of N >= 2304)
     static constexpr int G0 = PL::G ? PL::G : ((256 / T) > 16 ? 16 : ((256 / T) < 8 ? 8 : (256 / T)));
+    // LDS element: float2, or float for split exchanges (the tile is staged
+    // and read back one part at a time as well)
+    using E = typename PL::E;
+    static constexpr int pitch(int g)
+    {
+        return PL::SPLIT ? row_pitch_split(lds_len<N>(), g) : row_pitch(lds_len<N>(), g);
+    }
     static constexpr int fit_rows(int g)
     {
-        while (g > 1 && ((long)g * row_pitch(lds_len<N>(), g) * 8 > 160L * 1024 || g * T > 1024)) g /= 2;
+        while (g > 1 && ((long)g * pitch(g) * (long)sizeof(E) > 160L * 1024 || g * T > 1024)) g /= 2;
         return g;
     }
     static constexpr int G = fit_rows(G0);
-    static constexpr int L = row_pitch(lds_len<N>(), G);
+    static constexpr int L = pitch(G);
     static constexpr int THREADS = G * T;
     static constexpr int RFI = first_radix<PL, +1>();  // layout on entry of the inverse
     static constexpr int RLI = last_radix<PL, +1>();   // layout on exit of the inverse (= forward entry)
@@ -107,21 +134,25 @@ __device__ __forceinline__ void fetch_tile(float2* pre, const float2* __restrict
         }
     }
 }
-template <typename C>
-__device__ __forceinline__ void stage_tile(float2* lds, const float2* pre, int tid)
+template <typename C, int COMP = 0>
+__device__ __forceinline__ void stage_tile(typename Comp<COMP>::E* lds, const float2* pre, int tid)
 {
     if constexpr (C::T % 16 == 0) {
         int g, kx0;
         tile_elem<C>(tid, 0, g, kx0);
-        float2* dst = lds + g * C::L + lds_pad(kx0);
+        typename Comp<COMP>::E* dst = lds + g * C::L + lds_pad(kx0);
 #pragma unroll
-        for (int k = 0; k < C::P; ++k) dst[k * pad_c<C::T>()] = pre[k];
+        for (int k = 0; k < C::P; ++k) {
+            float2 x = pre[k];
+            dst[k * pad_c<C::T>()] = Comp<COMP>::of(x);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < C::P; ++k) {
             int g, kx;
             tile_elem<C>(tid, k, g, kx);
-            lds[g * C::L + lds_pad(kx)] = pre[k];
+            float2 x = pre[k];
+            lds[g * C::L + lds_pad(kx)] = Comp<COMP>::of(x);
         }
     }
 }
@@ -138,25 +169,34 @@ __device__ __forceinline__ float2 c2r_combine(float2 xk, float2 xmc, float2 w)
     return make_float2(a.x - b.y, a.y + b.x);
 }
 
+// exp(-2 pi i (p + c) / W) for a compile-time c, from w0 = exp(-2 pi i p / W)
+// (one load per thread) times the constant exp(-2 pi i c / W): the tables'
+// loads, not their arithmetic, bound the row passes.
+template <int N, int c> __device__ __forceinline__ float2 twiddle_w(float2 w0)
+{
+    if constexpr (c == 0) return w0;
+    else return cmul(w0, ct_tw<-1, (long)c, 2L * N>());
+}
+
 template <typename C>
 __device__ __forceinline__ void c2r_pre(float2* v, const float2* row, int t, const float2* __restrict__ twW)
 {
     using PL = typename C::PL;
     if constexpr (affine_layout<C, C::RFI>()) {
         const int pt = lds_pad(t), mt = lds_mirror(t);
-#pragma unroll
-        for (int i = 0; i < C::P; ++i) {
+        const float2 w0 = twW[t];
+        static_for<0, C::P>([&]<int i>() {
             constexpr int M = C::N / C::RFI;
-            const int c = (i / C::RFI) * C::T + (i % C::RFI) * M;  // slot_pos = t + c
+            constexpr int c = (i / C::RFI) * C::T + (i % C::RFI) * M;  // slot_pos = t + c
             const float2 xk = row[pt + c + c / 16];
-            if (i == 0) {
+            if constexpr (i == 0) {
                 const int im = t == 0 ? pt : C::N + C::N / 16 + mt;
-                const float2 z = c2r_combine(xk, row[im], twW[t]);
+                const float2 z = c2r_combine(xk, row[im], w0);
                 v[i] = t == 0 ? make_float2(xk.x + xk.y, xk.x - xk.y) : z;
             } else {
-                v[i] = c2r_combine(xk, row[(C::N - c) + (C::N - c) / 16 + mt], twW[t + c]);
+                v[i] = c2r_combine(xk, row[(C::N - c) + (C::N - c) / 16 + mt], twiddle_w<C::N, c>(w0));
             }
-        }
+        });
     } else {
 #pragma unroll
         for (int i = 0; i < C::P; ++i) {
@@ -166,6 +206,45 @@ __device__ __forceinline__ void c2r_pre(float2* v, const float2* row, int t, con
             else v[i] = c2r_combine(xk, row[lds_pad(C::N - p)], twW[p]);
         }
     }
+}
+
+// Split exchanges: stage the tile that v holds (as fetch_tile loaded it) one
+// part at a time and form the same inverse-FFT input as stage_tile + c2r_pre,
+// with the same operations: after the real parts, a.x = re X[p] + re X[N-p]
+// and d.x = re X[p] - re X[N-p]; after the imaginary parts, a.y and d.y; then
+// Z'[p] = (a.x - b.y, a.y + b.x), b = d * conj(w).
+template <typename C>
+__device__ __forceinline__ void stage_c2r_pre_split(float2* v, float* lds, int tid, int seq, int t,
+                                                    const float2* __restrict__ twW)
+{
+    static_assert(affine_layout<C, C::RFI>() && C::T % 16 == 0, "split exchanges need the affine layout");
+    constexpr int M = C::N / C::RFI;
+    const float* row = lds + seq * C::L;
+    const int pt = lds_pad(t), mt = lds_mirror(t);
+    float ax[C::P], dx[C::P], x0 = 0.0f;
+    stage_tile<C, 1>(lds, v, tid);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
+        const int c = (i / C::RFI) * C::T + (i % C::RFI) * M;
+        const float xk = row[pt + c + c / 16];
+        const float xm = i == 0 ? row[t == 0 ? pt : C::N + C::N / 16 + mt] : row[(C::N - c) + (C::N - c) / 16 + mt];
+        if (i == 0) x0 = xk;
+        ax[i] = xk + xm;
+        dx[i] = xk - xm;
+    }
+    __syncthreads();
+    stage_tile<C, 2>(lds, v, tid);
+    const float2 w0 = twW[t];
+    __syncthreads();
+    static_for<0, C::P>([&]<int i>() {
+        constexpr int c = (i / C::RFI) * C::T + (i % C::RFI) * M;
+        const float yk = row[pt + c + c / 16];
+        const float ym = i == 0 ? row[t == 0 ? pt : C::N + C::N / 16 + mt] : row[(C::N - c) + (C::N - c) / 16 + mt];
+        const float2 b = cmulc(make_float2(dx[i], yk + ym), twiddle_w<C::N, c>(w0));
+        const float2 z = make_float2(ax[i] - b.y, (yk - ym) + b.x);
+        v[i] = (i == 0 && t == 0) ? make_float2(x0 + yk, x0 - yk) : z;
+    });
 }
 
 // R2C post-processing and transposed store: v holds Z (forward FFT of the
@@ -181,11 +260,66 @@ __device__ __forceinline__ float2 r2c_combine(float2 zk, float2 zmc, float2 w, f
     return cadd(fe, cmul(w, fo));
 }
 
+// Split exchanges: the same, one part at a time, with the same operations
+// (s = Z[k] + conj Z[N-k] and d = Z[k] - conj Z[N-k] by parts).
 template <typename C>
-__device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, float2* __restrict__ X, int H,
+__device__ __forceinline__ void r2c_post_store_split(float2* v, float* lds, float2* __restrict__ X, int H, int y0,
+                                                     int tid, int seq, int t, const float2* __restrict__ twW,
+                                                     float scale)
+{
+    static_assert(affine_layout<C, C::RLF>() && C::T % 16 == 0, "split exchanges need the affine layout");
+    constexpr int M = C::N / C::RLF;
+    const float h = 0.5f * scale;
+    float* dst = lds + seq * C::L + lds_pad(t);
+    int g, kx0;
+    tile_elem<C>(fresh(tid), 0, g, kx0);
+    const float* r = lds + g * C::L;
+    const unsigned off0 = (unsigned)xpos(kx0, y0 + g, H), step = (unsigned)(C::T * H);
+    const int pk = lds_pad(kx0), mk = lds_mirror(kx0);
+    float sx[C::P], dx[C::P], z0 = 0.0f;
+    const float2 w0 = twW[kx0];
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
+        const int c = (i / C::RLF) * C::T + (i % C::RLF) * M;
+        dst[c + c / 16] = v[i].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::P; ++k) {
+        const int c = k * C::T;
+        const float zk = r[pk + c + c / 16];
+        const float zm = k == 0 ? r[kx0 == 0 ? pk : C::N + C::N / 16 + mk] : r[(C::N - c) + (C::N - c) / 16 + mk];
+        if (k == 0) z0 = zk;
+        sx[k] = zk + zm;
+        dx[k] = zk - zm;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < C::P; ++i) {
+        const int c = (i / C::RLF) * C::T + (i % C::RLF) * M;
+        dst[c + c / 16] = v[i].y;
+    }
+    __syncthreads();
+    static_for<0, C::P>([&]<int k>() {
+        constexpr int c = k * C::T;
+        const float zk = r[pk + c + c / 16];
+        const float zm = k == 0 ? r[kx0 == 0 ? pk : C::N + C::N / 16 + mk] : r[(C::N - c) + (C::N - c) / 16 + mk];
+        const float2 fe = make_float2(sx[k] * h, (zk - zm) * h);
+        const float2 fo = make_float2(h * (zk + zm), -h * dx[k]);
+        const float2 z = cadd(fe, cmul(twiddle_w<C::N, c>(w0), fo));
+        X[off0 + k * step] = (k == 0 && kx0 == 0) ? make_float2((z0 + zk) * scale, (z0 - zk) * scale) : z;
+    });
+}
+
+template <typename C>
+__device__ __forceinline__ void r2c_post_store(float2* v, typename C::E* lds, float2* __restrict__ X, int H,
                                                int y0, int tid, int seq, int t, const float2* __restrict__ twW,
                                                float scale)
 {
+    if constexpr (C::PL::SPLIT) {
+        r2c_post_store_split<C>(v, lds, X, H, y0, tid, seq, t, twW, scale);
+        return;
+    } else {
     using PL = typename C::PL;
     const float h = 0.5f * scale;
     float2* row = lds + seq * C::L;
@@ -204,24 +338,24 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
     __syncthreads();
     if constexpr (C::T % 16 == 0) {
         int g, kx0;
-        tile_elem<C>(tid, 0, g, kx0);
+        tile_elem<C>(fresh(tid), 0, g, kx0);
         const float2* r = lds + g * C::L;
         const unsigned off0 = (unsigned)xpos(kx0, y0 + g, H), step = (unsigned)(C::T * H);
         const int pk = lds_pad(kx0), mk = lds_mirror(kx0);
-#pragma unroll
-        for (int k = 0; k < C::P; ++k) {
-            const int c = k * C::T;
+        const float2 w0 = twW[kx0];
+        static_for<0, C::P>([&]<int k>() {
+            constexpr int c = k * C::T;
             const float2 zk = r[pk + c + c / 16];
             float2 xo;
-            if (k == 0) {
+            if constexpr (k == 0) {
                 const int im = kx0 == 0 ? pk : C::N + C::N / 16 + mk;
-                const float2 z = r2c_combine(zk, r[im], twW[kx0], h);
+                const float2 z = r2c_combine(zk, r[im], w0, h);
                 xo = kx0 == 0 ? make_float2((zk.x + zk.y) * scale, (zk.x - zk.y) * scale) : z;
             } else {
-                xo = r2c_combine(zk, r[(C::N - c) + (C::N - c) / 16 + mk], twW[kx0 + c], h);
+                xo = r2c_combine(zk, r[(C::N - c) + (C::N - c) / 16 + mk], twiddle_w<C::N, c>(w0), h);
             }
             X[off0 + k * step] = xo;
-        }
+        });
     } else {
 #pragma unroll 4
         for (int k = 0; k < C::P; ++k) {
@@ -233,6 +367,7 @@ __device__ __forceinline__ void r2c_post_store(const float2* v, float2* lds, flo
                                       : r2c_combine(zk, r[lds_pad(C::N - kx)], twW[kx], h);
             X[(unsigned)xpos(kx, y0 + g, H)] = xo;
         }
+    }
     }
 }
 
@@ -282,13 +417,21 @@ __device__ __forceinline__ void unpack_pair(PairT w, float& a, float& b)
     }
 }
 
-// Inverse real FFT of this thread's row from the staged tile (after the
-// workgroup barrier): on return v holds (f[2n], f[2n+1]) at the positions
-// slot_pos<PL, RLI>.
+// Inverse real FFT of this thread's row from the tile that v holds (as
+// fetch_tile loaded it): the tile is staged in LDS, the inverse FFT's input
+// formed from it and transformed; on return v holds (f[2n], f[2n+1]) at the
+// positions slot_pos<PL, RLI>.
 template <typename C>
-__device__ __forceinline__ void row_c2r(float2* v, float2* row, int t, const RowGeom& g)
+__device__ __forceinline__ void row_c2r(float2* v, typename C::E* lds, int tid, int seq, int t, const RowGeom& g)
 {
-    c2r_pre<C>(v, row, t, g.twW);
+    typename C::E* row = lds + seq * C::L;
+    if constexpr (C::PL::SPLIT) {
+        stage_c2r_pre_split<C>(v, lds, tid, seq, t, g.twW);
+    } else {
+        stage_tile<C>(lds, v, tid);
+        __syncthreads();
+        c2r_pre<C>(v, row, t, g.twW);
+    }
     seq_sync<typename C::PL>();
     fft_regs<typename C::PL, +1>(v, row, t, g.twP[C::PL::variant]);
 }
@@ -304,13 +447,14 @@ __device__ __forceinline__ void row_c2r(float2* v, float2* row, int t, const Row
 // is 1, or 1/W when pass C's kernel spectrum is pass A's (which carries 1/H only).
 template <typename SAMPLE, typename PL>
 __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict__ Xp,
-                                           const SAMPLE* __restrict__ dplane, int y0, float2* lds, float scale)
+                                           const SAMPLE* __restrict__ dplane, int y0, typename PL::E* lds,
+                                           float scale)
 {
     using C = RowCfg<PL>;
     using PairT = typename PairOf<SAMPLE>::type;
     const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
     const int y = y0 + seq;
-    float2* row = lds + seq * C::L;
+    typename C::E* row = lds + seq * C::L;
     float2 v[C::P];
     fetch_tile<C>(v, Xp, g.H, y0, tid);
     // the row's samples (streamed from HBM once per view-iteration) are loaded
@@ -318,14 +462,18 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
     const SAMPLE* drow = dplane + (unsigned)(pad_index(y, g.dataH, g.dkY) * g.dataW);
     const bool pairs = (g.dataW % 2 == 0) && ((uintptr_t)dplane % (2 * sizeof(SAMPLE)) == 0);
     PairT d[C::P];
+    auto load_samples = [&]() {
 #pragma unroll
-    for (int i = 0; i < C::P; ++i) {
-        const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
-        d[i] = load_padded_pair<SAMPLE>(drow, t + c, c, C::T, g.dataW, g.dkX, pairs);
-    }
-    stage_tile<C>(lds, v, tid);
-    __syncthreads();
-    row_c2r<C>(v, row, t, g);
+        for (int i = 0; i < C::P; ++i) {
+            const int c = (i / C::RLI) * C::T + (i % C::RLI) * (C::N / C::RLI);
+            d[i] = load_padded_pair<SAMPLE>(drow, t + c, c, C::T, g.dataW, g.dkX, pairs);
+        }
+    };
+    // split plans run at 4 waves per SIMD within 128 registers, which leave
+    // no room to carry the samples through the inverse FFT: they load after it
+    if constexpr (!PL::SPLIT) load_samples();
+    row_c2r<C>(v, lds, tid, seq, t, g);
+    if constexpr (PL::SPLIT) load_samples();
 #pragma unroll
     for (int i = 0; i < C::P; ++i) {
         const float q0 = v[i].x, q1 = v[i].y;
@@ -346,17 +494,15 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
 template <typename PL>
 __device__ __forceinline__ void update_tile(const RowGeom& g, float2* XEp,
                                             const float2* XCp, float* __restrict__ ESp,
-                                            const float* __restrict__ weights, int y0, float2* lds)
+                                            const float* __restrict__ weights, int y0, typename PL::E* lds)
 {
     using C = RowCfg<PL>;
     const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
     const int y = y0 + seq;
-    float2* row = lds + seq * C::L;
+    typename C::E* row = lds + seq * C::L;
     float2 v[C::P];
     fetch_tile<C>(v, XCp, g.H, y0, tid);
-    stage_tile<C>(lds, v, tid);
-    __syncthreads();
-    row_c2r<C>(v, row, t, g);
+    row_c2r<C>(v, lds, tid, seq, t, g);
 
     float2* erow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
     const float* wrow = weights + (unsigned)(pad_index(y, g.dataH, g.wkY) * g.dataW);
@@ -383,12 +529,12 @@ __device__ __forceinline__ void update_tile(const RowGeom& g, float2* XEp,
 // Warm-start ingest: ES <- pad(E0) (data pad map), XE <- R2C(ES) / W.
 template <typename PL>
 __device__ __forceinline__ void init_tile(const RowGeom& g, float2* __restrict__ XEp, float* __restrict__ ESp,
-                                          const float* __restrict__ e0p, int y0, float2* lds)
+                                          const float* __restrict__ e0p, int y0, typename PL::E* lds)
 {
     using C = RowCfg<PL>;
     const int tid = threadIdx.x, seq = tid / C::T, t = tid % C::T;
     const int y = y0 + seq;
-    float2* row = lds + seq * C::L;
+    typename C::E* row = lds + seq * C::L;
     const float* erow = e0p + (unsigned)(pad_index(y, g.dataH, g.dkY) * g.dataW);
     float2* esrow = reinterpret_cast<float2*>(ESp) + (unsigned)(y * (g.W / 2));
     float2 v[C::P];
@@ -417,12 +563,13 @@ template <typename PL_> struct ColCfg {
     static constexpr int H = PL::N;
     static constexpr int T = PL::T;
     static constexpr int P = PL::P;
+    using E = typename PL::E;  // LDS element (float for split exchanges)
     static constexpr int L = lds_len<H>() + 1;
     // sequences per workgroup (a power of two, so it divides the row count):
     // aim for 256 threads, keep LDS <= ~64 KB
     static constexpr int SQ0 = (256 / T) < 1 ? 1 : (256 / T);
     static constexpr int SQ1 = SQ0 > 16 ? 16 : SQ0;
-    static constexpr int SQ2 = 65536 / (L * 8) < 1 ? 1 : 65536 / (L * 8);
+    static constexpr int SQ2 = 65536 / (L * (int)sizeof(E)) < 1 ? 1 : 65536 / (L * (int)sizeof(E));
     static constexpr int SQ = PL::G ? PL::G : pow2_floor(SQ1 < SQ2 ? SQ1 : SQ2);
     static constexpr int THREADS = SQ * T;
     static constexpr int RF = first_radix<PL, -1>();
@@ -446,6 +593,55 @@ __device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ K
     }
 }
 
+// Split exchanges, the workgroup of kx = 0: sequence 0 holds the packed
+// DC/Nyquist spectrum C and forms A*K0 + i*B*KN from it with the same
+// operations as col_tile, through two float rows at once: its own row takes
+// the real parts, sequence 1's row (idle until that sequence's inverse FFT)
+// the imaginary parts.  The other sequences multiply by their kernel row.
+// The workgroup barriers order the borrowed row against sequence 1's own use
+// of it; grp is uniform over the workgroup and no sequence has left.
+template <typename PL>
+__device__ __forceinline__ void col_dc_nyquist_split(const ColGeom& g, float2* v, float* row,
+                                                     const float2* __restrict__ K, int seq, int t, int kx,
+                                                     float ksign)
+{
+    using C = ColCfg<PL>;
+    constexpr int H = C::H;
+    static_assert(C::SQ >= 2 && C::SQ <= 8, "split column plans hold 2 to 8 sequences per workgroup");
+    float* rim = row + C::L;
+    __syncthreads();
+    if (seq == 0) {
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) {
+            const int k = lds_pad(slot_pos<PL, C::RL>(t, i));
+            row[k] = v[i].x;
+            rim[k] = v[i].y;
+        }
+    }
+    __syncthreads();
+    if (seq == 0) {
+        const float2* K0 = K;                         // kx = 0, stride 2
+        const float2* KN = K + (unsigned)(g.N * H);  // kx = N, contiguous
+#pragma unroll
+        for (int i = 0; i < C::P; ++i) {
+            const int k = slot_pos<PL, C::RL>(t, i);
+            const int pk = lds_pad(k), pm = lds_pad(k == 0 ? 0 : H - k);
+            const float2 ck = make_float2(row[pk], rim[pk]);
+            const float2 cm = cconj(make_float2(row[pm], rim[pm]));
+            const float2 a = cscale(cadd(ck, cm), 0.5f);
+            const float2 d = csub(ck, cm);
+            const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
+            const float2 k0 = K0[2 * k], kn = KN[k];
+            const float2 ak = cmul(a, make_float2(k0.x, ksign * k0.y));
+            const float2 bk = cmul(b, make_float2(kn.x, ksign * kn.y));
+            v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
+        }
+    } else {
+        col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
+    }
+    __syncthreads();
+}
+
 // Passes A / C on the kx-rows [grp*SQ, grp*SQ+SQ) of one plane:
 // Xout[kx] <- IFFT_y(FFT_y(Xin[kx]) * K[kx]); Xout may be Xin (each wave holds
 // its whole sequence in registers before it stores).  The cuFFT column halves plus
@@ -459,7 +655,7 @@ __device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ K
 // C can read pass A's K.
 template <typename PL>
 __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, float2* Xout,
-                                         const float2* __restrict__ K, int grp, float2* lds, float ksign)
+                                         const float2* __restrict__ K, int grp, typename PL::E* lds, float ksign)
 {
     using C = ColCfg<PL>;
     constexpr int H = C::H;
@@ -472,14 +668,17 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
     // their wavefront, so the overhanging ones just leave
     if (kx >= g.N) return;
     const float2* xin = Xin + (unsigned)xpos(kx, 0, H);  // stride 2 along y
-    float2* row = lds + seq * C::L;
+    typename C::E* row = lds + seq * C::L;
 
     float2 v[C::P];
 #pragma unroll
     for (int i = 0; i < C::P; ++i) v[i] = xin[2 * slot_pos<PL, C::RF>(t, i)];
     fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
 
-    if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
+    if constexpr (PL::SPLIT) {
+        if (grp == 0) col_dc_nyquist_split<PL>(g, v, row, K, seq, t, kx, ksign);
+        else col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
+    } else if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
         if (seq == 0) {
 #pragma unroll
             for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RL>(t, i))] = v[i];
@@ -511,8 +710,9 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
 
     fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);
     float2* xo = Xout + (unsigned)xpos(kx, 0, H);
+    const int ts = fresh(t);
 #pragma unroll
-    for (int i = 0; i < C::P; ++i) xo[2 * slot_pos<PL, C::RF>(t, i)] = v[i];
+    for (int i = 0; i < C::P; ++i) xo[2 * slot_pos<PL, C::RF>(ts, i)] = v[i];
 }
 
 }  // namespace fmvd

@@ -23,19 +23,68 @@ template <typename PL> struct Emu {
     static constexpr int N = PL::N, T = PL::T, P = PL::P;
     std::vector<float2> tw = pass_twiddle_table<PL>();
     std::vector<float2> lds = std::vector<float2>(lds_len<N>() + 2 * kGuard);
+    std::vector<float> ldsf = std::vector<float>(lds_len<N>() + 2 * kGuard);  // split exchanges
     std::vector<float2> v = std::vector<float2>((size_t)T * P);
     float2* L() { return lds.data() + kGuard; }
+    float* LF() { return ldsf.data() + kGuard; }
     float2* V(int t) { return v.data() + (size_t)t * P; }
+    bool clean = true;
 
     void reset_lds()
     {
         for (auto& x : lds) x = make_float2(NAN, NAN);
+        for (auto& x : ldsf) x = NAN;
     }
     bool guards_clean()
     {
         for (int i = 0; i < kGuard; ++i)
-            if (!std::isnan(lds[i].x) || !std::isnan(lds[lds.size() - 1 - i].x)) return false;
-        return true;
+            if (!std::isnan(lds[i].x) || !std::isnan(lds[lds.size() - 1 - i].x) || !std::isnan(ldsf[i]) ||
+                !std::isnan(ldsf[ldsf.size() - 1 - i]))
+                return false;
+        return clean;
+    }
+
+    // The exchanges of fwd_exchange / inv_exchange: all T threads write, then
+    // all read; split plans move the real parts, then the imaginary parts,
+    // through the float buffer (the buffer is refilled with NaN before each
+    // part, so a read of a slot the part did not write shows up as NaN).
+    template <int I, int COMP> void fwd_part()
+    {
+        reset_lds();
+        for (int t = 0; t < T; ++t) lds_out_slots<PL, I - 1, true, COMP>(LF(), V(t), t);
+        for (int t = 0; t < T; ++t) lds_in_slots<PL, I, false, COMP>(LF(), V(t), t);
+        clean = clean && guards_clean();
+    }
+    template <int I, int COMP> void inv_part()
+    {
+        reset_lds();
+        for (int t = 0; t < T; ++t) lds_in_slots<PL, I + 1, true, COMP>(LF(), V(t), t);
+        for (int t = 0; t < T; ++t) lds_out_slots<PL, I, false, COMP>(LF(), V(t), t);
+        clean = clean && guards_clean();
+    }
+    template <int I> void fwd_exchange_emu()
+    {
+        if constexpr (PL::SPLIT) {
+            fwd_part<I, 1>();
+            fwd_part<I, 2>();
+        } else {
+            reset_lds();
+            for (int t = 0; t < T; ++t) lds_out_slots<PL, I - 1, true>(L(), V(t), t);
+            for (int t = 0; t < T; ++t) lds_in_slots<PL, I, false>(L(), V(t), t);
+            clean = clean && guards_clean();
+        }
+    }
+    template <int I> void inv_exchange_emu()
+    {
+        if constexpr (PL::SPLIT) {
+            inv_part<I, 1>();
+            inv_part<I, 2>();
+        } else {
+            reset_lds();
+            for (int t = 0; t < T; ++t) lds_in_slots<PL, I + 1, true>(L(), V(t), t);
+            for (int t = 0; t < T; ++t) lds_out_slots<PL, I, false>(L(), V(t), t);
+            clean = clean && guards_clean();
+        }
     }
 
     template <int I> void fwd()
@@ -45,9 +94,7 @@ template <typename PL> struct Emu {
                 // as fwd_passes: the twiddles are loaded before the exchange
                 std::vector<float2> w((size_t)T * tw_regs<PL, I>());
                 for (int t = 0; t < T; ++t) load_twiddles<PL, I>(w.data() + (size_t)t * tw_regs<PL, I>(), t, tw.data());
-                reset_lds();
-                for (int t = 0; t < T; ++t) lds_out_slots<PL, I - 1, true>(L(), V(t), t);
-                for (int t = 0; t < T; ++t) lds_in_slots<PL, I, false>(L(), V(t), t);
+                fwd_exchange_emu<I>();
                 for (int t = 0; t < T; ++t) apply_twiddles<PL, I, false>(V(t), w.data() + (size_t)t * tw_regs<PL, I>());
             } else {
                 for (int t = 0; t < T; ++t) pass_twiddle<PL, I, false>(V(t), t, tw.data());
@@ -64,11 +111,7 @@ template <typename PL> struct Emu {
             std::vector<float2> w((size_t)T * tw_regs<PL, I>());
             if constexpr (pre)
                 for (int t = 0; t < T; ++t) load_twiddles<PL, I>(w.data() + (size_t)t * tw_regs<PL, I>(), t, tw.data());
-            if constexpr (I < PL::NP - 1) {
-                reset_lds();
-                for (int t = 0; t < T; ++t) lds_in_slots<PL, I + 1, true>(L(), V(t), t);
-                for (int t = 0; t < T; ++t) lds_out_slots<PL, I, false>(L(), V(t), t);
-            }
+            if constexpr (I < PL::NP - 1) inv_exchange_emu<I>();
             for (int t = 0; t < T; ++t) {
                 pass_dft<PL, I, +1>(V(t));
                 if constexpr (pre) apply_twiddles<PL, I, true>(V(t), w.data() + (size_t)t * tw_regs<PL, I>());
@@ -134,8 +177,8 @@ template <typename PL> void check(const char* name)
     const double tol = 2e-6 * std::log2((double)PL::N);
     const bool ok = worst < tol && guards;
     if (!ok) ++failures;
-    printf("%-26s N=%5d T=%4d P=%3d passes=%d relerr=%.2e guards=%s %s\n", name, PL::N, PL::T, PL::P, PL::NP, worst,
-           guards ? "clean" : "TOUCHED", ok ? "ok" : "FAIL");
+    printf("%-26s N=%5d T=%4d P=%3d passes=%d%s relerr=%.2e guards=%s %s\n", name, PL::N, PL::T, PL::P, PL::NP,
+           PL::SPLIT ? " split" : "", worst, guards ? "clean" : "TOUCHED", ok ? "ok" : "FAIL");
 }
 
 #define CHECK_ALL(n)                                                             \
