@@ -250,20 +250,18 @@ template <int R, int DIR> struct DftOddPrime {
             sum = cadd(sum, b[j]);
         }
         v[0] = sum;
-#pragma unroll
-        for (int k = 1; k <= H; ++k) {
+        static_for<1, H + 1>([&]<int k>() {
             float2 m = a, e = make_float2(0.0f, 0.0f);
-#pragma unroll
-            for (int j = 1; j <= H; ++j) {
-                const float c = (float)ct_cos2pi((long)j * k % R, R);
-                const float sn = (float)(DIR * ct_sin2pi((long)j * k % R, R));
+            static_for<1, H + 1>([&]<int j>() {
+                constexpr float c = (float)ct_cos2pi((long)j * k % R, R);
+                constexpr float sn = (float)(DIR * ct_sin2pi((long)j * k % R, R));
                 m = make_float2(m.x + c * b[j].x, m.y + c * b[j].y);
                 e = make_float2(e.x + sn * d[j].x, e.y + sn * d[j].y);
-            }
+            });
             // i * e = (-e.y, e.x)
             v[k] = make_float2(m.x - e.y, m.y + e.x);
             v[R - k] = make_float2(m.x + e.y, m.y - e.x);
-        }
+        });
     }
 };
 template <int DIR> struct Dft<11, DIR> : DftOddPrime<11, DIR> {};
@@ -276,18 +274,16 @@ template <int A, int B, int DIR> struct DftSplit {
     {
         constexpr int R = A * B;
         float2 t[R];
-#pragma unroll
-        for (int n2 = 0; n2 < B; ++n2) {
+        static_for<0, B>([&]<int n2>() {
             float2 s[A];
 #pragma unroll
             for (int n1 = 0; n1 < A; ++n1) s[n1] = v[B * n1 + n2];
             Dft<A, DIR>::run(s);
-#pragma unroll
-            for (int k1 = 0; k1 < A; ++k1) {
-                if (n2 * k1 == 0) t[k1 * B + n2] = s[k1];
-                else t[k1 * B + n2] = cmul(s[k1], ct_twiddle<DIR>(n2 * k1, R));
-            }
-        }
+            static_for<0, A>([&]<int k1>() {
+                if constexpr (n2 * k1 == 0) t[k1 * B + n2] = s[k1];
+                else t[k1 * B + n2] = cmul(s[k1], ct_tw<DIR, (long)n2 * k1, (long)R>());
+            });
+        });
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) {
             float2 s[B];
@@ -359,7 +355,7 @@ template <> struct Plan<256> : PlanDef<256, 16, RadixList<16, 16>> {};
 template <> struct Plan<512> : PlanDef<512, 32, RadixList<16, 16, 2>> {};
 template <> struct Plan<768> : PlanDef<768, 64, RadixList<4, 4, 4, 12>, 8, 4> {};
 template <> struct Plan<1024> : PlanDef<1024, 64, RadixList<16, 16, 4>> {};
-template <> struct Plan<1280> : PlanDef<1280, 64, RadixList<4, 4, 4, 20>> {};
+template <> struct Plan<1280> : PlanDef<1280, 64, RadixList<4, 4, 4, 20>, 8, 4, true> {};
 template <> struct Plan<1536> : PlanDef<1536, 64, RadixList<8, 8, 24>> {};
 template <> struct Plan<1792> : PlanDef<1792, 64, RadixList<4, 4, 4, 28>> {};
 template <> struct Plan<2048> : PlanDef<2048, 128, RadixList<16, 16, 8>> {};
@@ -606,9 +602,10 @@ FMVD_HD void lds_out_slots(typename Comp<COMP>::E* lds, float2* v, int t)
 // loads w^k and, for radices above 8, w^(Q k) with Q = ceil(sqrt(R)) from the
 // pass's table, and forms w^(k r) = w^(k a) w^(Q k m) (r = a + Q m) with the
 // powers built by binary splitting, so at most five rounded products deep.
-// A butterfly b whose k is b = 0's plus a compile-time d_b further multiplies
-// by the constant w^(d_b r).  One or two loads per pass replace NB (R - 1):
-// the loads' data return, not the arithmetic, bounds the passes on gfx950.
+// Butterflies whose k differ (k_b = k_0 + d_b, d_b a compile-time offset)
+// load their own pair of powers.  One or two loads per distinct k replace
+// NB (R - 1): the loads' data return, not the arithmetic, bounds the passes
+// on gfx950.
 template <int R> constexpr int tw_gen_q()
 {
     if (R <= 8) return R;
@@ -617,11 +614,18 @@ template <int R> constexpr int tw_gen_q()
     return q;
 }
 template <typename PL, int I> constexpr bool tw_generated() { return span_before<PL>(I) > 1 && affine_tw<PL, I>(); }
+// Whether the butterflies of a generated pass share one k (d_b = 0 for all b).
+template <typename PL, int I> constexpr bool tw_shared_k()
+{
+    for (int b = 0; b < PL::P / PL::R::at(I); ++b)
+        if (tw_k<PL, I>(0, b) != 0) return false;
+    return true;
+}
 
 // Registers a pass's twiddles occupy between their load and their use.
 template <typename PL, int I> constexpr int tw_regs()
 {
-    if constexpr (tw_generated<PL, I>()) return 2;
+    if constexpr (tw_generated<PL, I>()) return tw_shared_k<PL, I>() ? 2 : 2 * (PL::P / PL::R::at(I));
     else return span_before<PL>(I) > 1 ? (PL::P / PL::R::at(I)) * (PL::R::at(I) - 1) : 1;
 }
 
@@ -633,10 +637,13 @@ FMVD_HD void load_twiddles(float2* w, int t, const float2* __restrict__ tw)
     if constexpr (NS > 1) {
         const float2* twp = tw + tw_pass_offset<PL>(I);
         if constexpr (tw_generated<PL, I>()) {
-            constexpr int Q = tw_gen_q<R>();
+            constexpr int Q = tw_gen_q<R>(), NK = tw_shared_k<PL, I>() ? 1 : NB;
             const float2* tb = twp + tw_k<PL, I>(t, 0);
-            w[0] = tb[0];                                    // w^k (r = 1)
-            if constexpr (Q < R) w[1] = tb[(Q - 1) * NS];    // w^(Q k) (r = Q)
+            static_for<0, NK>([&]<int b>() {
+                constexpr int d = tw_k<PL, I>(0, b);
+                w[2 * b] = tb[d];                                    // w^k (r = 1)
+                if constexpr (Q < R) w[2 * b + 1] = tb[d + (Q - 1) * NS];  // w^(Q k) (r = Q)
+            });
         } else {
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -655,23 +662,25 @@ template <typename PL, int I, bool CONJ> FMVD_HD void apply_twiddles(float2* v, 
     if constexpr (NS > 1) {
         if constexpr (tw_generated<PL, I>()) {
             constexpr int Q = tw_gen_q<R>(), M = (R + Q - 1) / Q;
+            constexpr bool shared = tw_shared_k<PL, I>();
             float2 pa[Q], pm[M > 1 ? M : 1];
-            pa[1] = w[0];
-#pragma unroll
-            for (int n = 2; n < Q; ++n) pa[n] = cmul(pa[n / 2], pa[n - n / 2]);
-            if constexpr (M > 1) {
-                pm[1] = w[1];
-#pragma unroll
-                for (int n = 2; n < M; ++n) pm[n] = cmul(pm[n / 2], pm[n - n / 2]);
-            }
             static_for<0, NB>([&]<int b>() {
+                if constexpr (b == 0 || !shared) {
+                    pa[1] = w[2 * b];
+#pragma unroll
+                    for (int n = 2; n < Q; ++n) pa[n] = cmul(pa[n / 2], pa[n - n / 2]);
+                    if constexpr (M > 1) {
+                        pm[1] = w[2 * b + 1];
+#pragma unroll
+                        for (int n = 2; n < M; ++n) pm[n] = cmul(pm[n / 2], pm[n - n / 2]);
+                    }
+                }
                 static_for<1, R>([&]<int r>() {
-                    constexpr int a = r % Q, m = r / Q, d = tw_k<PL, I>(0, b);
+                    constexpr int a = r % Q, m = r / Q;
                     float2 x;
                     if constexpr (a == 0) x = pm[m];
                     else if constexpr (m == 0) x = pa[a];
                     else x = cmul(pa[a], pm[m]);
-                    if constexpr (d != 0) x = cmul(x, ct_tw<-1, (long)d * r, (long)R * NS>());
                     v[b * R + r] = CONJ ? cmulc(v[b * R + r], x) : cmul(v[b * R + r], x);
                 });
             });
@@ -767,45 +776,47 @@ __device__ __forceinline__ void inv_exchange(typename PL::E* lds, float2* v, int
     }
 }
 
+// Where a pass's twiddles come from: the plan's per-pass table, read as
+// each pass needs it.
+struct TwTable {
+    const float2* __restrict__ tw;
+};
+template <typename PL, int I> FMVD_HD void tw_get(float2* w, int t, const TwTable& src)
+{
+    load_twiddles<PL, I>(w, t, src.tw);
+}
+
 // Forward passes I..NP-1; on entry v holds pass I-1's outputs (or, for I = 0,
 // the inputs at the "j + r*N/R0" layout).
-template <typename PL, int I>
-__device__ __forceinline__ void fwd_passes(float2* v, typename PL::E* lds, int t, const float2* __restrict__ tw)
+template <typename PL, int I, typename TS>
+__device__ __forceinline__ void fwd_passes(float2* v, typename PL::E* lds, int t, const TS& ts)
 {
     if constexpr (I < PL::NP) {
-        if constexpr (I > 0) {
-            // this pass's twiddles load while the exchange runs
-            float2 w[tw_regs<PL, I>()];
-            load_twiddles<PL, I>(w, t, tw);
-            fwd_exchange<PL, I>(lds, v, t);
-            apply_twiddles<PL, I, false>(v, w);
-        } else {
-            pass_twiddle<PL, I, false>(v, t, tw);
-        }
+        // this pass's twiddles load while the exchange runs
+        float2 w[tw_regs<PL, I>()];
+        tw_get<PL, I>(w, t, ts);
+        if constexpr (I > 0) fwd_exchange<PL, I>(lds, v, t);
+        apply_twiddles<PL, I, false>(v, w);
         pass_dft<PL, I, -1>(v);
-        fwd_passes<PL, I + 1>(v, lds, t, tw);
+        fwd_passes<PL, I + 1>(v, lds, t, ts);
     }
 }
 // Inverse passes I..0 (transposed forward passes, run backwards); on entry v
-// holds the values at pass I's "out" slots.
-template <typename PL, int I>
-__device__ __forceinline__ void inv_passes(float2* v, typename PL::E* lds, int t, const float2* __restrict__ tw)
+// holds the values at pass I's "out" slots.  Small-radix passes load their
+// twiddles before the exchange (they are applied after the DFT, whose few
+// temporaries leave room); larger ones after the DFT.
+template <typename PL, int I, typename TS>
+__device__ __forceinline__ void inv_passes(float2* v, typename PL::E* lds, int t, const TS& ts)
 {
     if constexpr (I >= 0) {
-        if constexpr (I < PL::NP - 1 && PL::R::at(I) <= 4) {
-            // small-radix passes: the twiddles load while the exchange runs
-            // (they are applied after the DFT, whose few temporaries leave room)
-            float2 w[tw_regs<PL, I>()];
-            load_twiddles<PL, I>(w, t, tw);
-            inv_exchange<PL, I>(lds, v, t);
-            pass_dft<PL, I, +1>(v);
-            apply_twiddles<PL, I, true>(v, w);
-        } else {
-            if constexpr (I < PL::NP - 1) inv_exchange<PL, I>(lds, v, t);
-            pass_dft<PL, I, +1>(v);
-            pass_twiddle<PL, I, true>(v, t, tw);
-        }
-        inv_passes<PL, I - 1>(v, lds, t, tw);
+        constexpr bool early = I < PL::NP - 1 && PL::R::at(I) <= 4;
+        float2 w[tw_regs<PL, I>()];
+        if constexpr (early) tw_get<PL, I>(w, t, ts);
+        if constexpr (I < PL::NP - 1) inv_exchange<PL, I>(lds, v, t);
+        pass_dft<PL, I, +1>(v);
+        if constexpr (!early) tw_get<PL, I>(w, t, ts);
+        apply_twiddles<PL, I, true>(v, w);
+        inv_passes<PL, I - 1>(v, lds, t, ts);
     }
 }
 
@@ -814,12 +825,12 @@ __device__ __forceinline__ void inv_passes(float2* v, typename PL::E* lds, int t
 // "j + r*N/R" of R = first_radix<PL, DIR>() (slot_pos); on exit the outputs,
 // in natural order, at the layout of last_radix<PL, DIR>().  The caller must
 // have ordered any earlier LDS reads of this sequence's slice before the
-// call (seq_sync or a workgroup barrier).
+// call (seq_sync or a workgroup barrier).  tw: the plan's twiddle table.
 template <typename PL, int DIR>
 __device__ __forceinline__ void fft_regs(float2* v, typename PL::E* lds, int t, const float2* __restrict__ tw)
 {
-    if constexpr (DIR < 0) fwd_passes<PL, 0>(v, lds, t, tw);
-    else inv_passes<PL, PL::NP - 1>(v, lds, t, tw);
+    if constexpr (DIR < 0) fwd_passes<PL, 0>(v, lds, t, TwTable{tw});
+    else inv_passes<PL, PL::NP - 1>(v, lds, t, TwTable{tw});
 }
 
 // Position of input r of butterfly b for thread t in the layout of radix R.

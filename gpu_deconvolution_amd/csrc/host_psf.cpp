@@ -253,10 +253,9 @@ std::vector<float2> kernel_spectrum_transposed(const double* k, int kw, int kh, 
             col[2 * yi + 1] += F[(size_t)i * (N + 1) + kx].imag();
         }
         run_fft(colplan, col);
-        // device layout (rl_launch.hpp): kx pairs interleaved, row N contiguous
+        // device layout (rl_launch.hpp): one contiguous column per kx
         for (int ky = 0; ky < H; ++ky)
-            out[kx < N ? (size_t)xpos(kx, ky, H) : (size_t)N * H + ky] =
-                make_float2((float)(col[2 * ky] * scale), (float)(col[2 * ky + 1] * scale));
+            out[(size_t)kx * H + ky] = make_float2((float)(col[2 * ky] * scale), (float)(col[2 * ky + 1] * scale));
     });
     return out;
 }

@@ -28,8 +28,8 @@ std::vector<float> normalized_weights(const float* const* w, int dw, int dh, int
 
 // Spectrum of a kh x kw kernel padded onto the H x W torus centred at (0,0)
 // with wraparound (fmvd_deconvolve_cuda.cu:13-38), multiplied by `scale`,
-// in the device layout of rl_launch.hpp: K[ky][kx] at xpos(kx, ky, H) for
-// kx < W/2 (kx pairs interleaved) and at (W/2) H + ky for kx = W/2.
+// in the device layout of rl_launch.hpp: K[ky][kx] at kx H + ky for
+// kx <= W/2 (one contiguous column per kx).
 std::vector<float2> kernel_spectrum_transposed(const double* k, int kw, int kh, int H, int W,
                                                double scale);
 

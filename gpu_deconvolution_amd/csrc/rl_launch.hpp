@@ -11,9 +11,10 @@
 // bandwidth); the row passes read and write it transposed in tiles of G rows,
 // whose runs are 2G complex long (128 bytes at G = 8, whole cache lines).
 //
-// Kernel spectra use the same layout for kx in [0, N) (kx = 0 holding
-// K[ky][0]) followed by the contiguous row kx = N (K[ky][N]) at N*fftH, all
-// complex and already multiplied by the pass's normalisation.
+// Kernel spectra hold one contiguous column per kx in [0, N] (K[ky][kx] at
+// kx*fftH + ky, kx = 0 holding K[ky][0] and kx = N the Nyquist column), all
+// complex and already multiplied by the pass's normalisation: the column
+// pass, their only reader, then loads them with unit stride.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

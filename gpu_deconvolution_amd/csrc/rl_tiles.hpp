@@ -469,11 +469,8 @@ __device__ __forceinline__ void ratio_tile(const RowGeom& g, float2* __restrict_
             d[i] = load_padded_pair<SAMPLE>(drow, t + c, c, C::T, g.dataW, g.dkX, pairs);
         }
     };
-    // split plans run at 4 waves per SIMD within 128 registers, which leave
-    // no room to carry the samples through the inverse FFT: they load after it
-    if constexpr (!PL::SPLIT) load_samples();
+    load_samples();
     row_c2r<C>(v, lds, tid, seq, t, g);
-    if constexpr (PL::SPLIT) load_samples();
 #pragma unroll
     for (int i = 0; i < C::P; ++i) {
         const float q0 = v[i].x, q1 = v[i].y;
@@ -579,8 +576,8 @@ template <typename PL_> struct ColCfg {
 };
 
 // v[i] *= Kr[y] at the inverse-entry positions y of thread t, with Kr[y]'s
-// imaginary part times ksign (-1: the conjugate); Kr is one kx-row of a
-// kernel spectrum (stride 2 along y).  The sign is a multiply, not a second
+// imaginary part times ksign (-1: the conjugate); Kr is one kx column of a
+// kernel spectrum (contiguous along y).  The sign is a multiply, not a second
 // code path, so every kernel that inlines this computes the same bits.
 template <typename PL>
 __device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ Kr, int t, float ksign)
@@ -588,7 +585,7 @@ __device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ K
     constexpr int RL = last_radix<PL, -1>();
 #pragma unroll
     for (int i = 0; i < PL::P; ++i) {
-        const float2 k = Kr[2 * slot_pos<PL, RL>(t, i)];
+        const float2 k = Kr[slot_pos<PL, RL>(t, i)];
         v[i] = cmul(v[i], make_float2(k.x, ksign * k.y));
     }
 }
@@ -620,7 +617,7 @@ __device__ __forceinline__ void col_dc_nyquist_split(const ColGeom& g, float2* v
     }
     __syncthreads();
     if (seq == 0) {
-        const float2* K0 = K;                         // kx = 0, stride 2
+        const float2* K0 = K;                         // kx = 0
         const float2* KN = K + (unsigned)(g.N * H);  // kx = N, contiguous
 #pragma unroll
         for (int i = 0; i < C::P; ++i) {
@@ -631,13 +628,13 @@ __device__ __forceinline__ void col_dc_nyquist_split(const ColGeom& g, float2* v
             const float2 a = cscale(cadd(ck, cm), 0.5f);
             const float2 d = csub(ck, cm);
             const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
-            const float2 k0 = K0[2 * k], kn = KN[k];
+            const float2 k0 = K0[k], kn = KN[k];
             const float2 ak = cmul(a, make_float2(k0.x, ksign * k0.y));
             const float2 bk = cmul(b, make_float2(kn.x, ksign * kn.y));
             v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
         }
     } else {
-        col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
+        col_kmul<PL>(v, K + (unsigned)(kx * H), t, ksign);
     }
     __syncthreads();
 }
@@ -677,7 +674,7 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
 
     if constexpr (PL::SPLIT) {
         if (grp == 0) col_dc_nyquist_split<PL>(g, v, row, K, seq, t, kx, ksign);
-        else col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
+        else col_kmul<PL>(v, K + (unsigned)(kx * H), t, ksign);
     } else if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
         if (seq == 0) {
 #pragma unroll
@@ -685,7 +682,7 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
         }
         seq_sync<PL>();
         if (seq == 0) {
-            const float2* K0 = K;                         // kx = 0, stride 2
+            const float2* K0 = K;                         // kx = 0
             const float2* KN = K + (unsigned)(g.N * H);  // kx = N, contiguous
 #pragma unroll
             for (int i = 0; i < C::P; ++i) {
@@ -695,17 +692,17 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
                 const float2 a = cscale(cadd(ck, cm), 0.5f);
                 const float2 d = csub(ck, cm);
                 const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
-                const float2 k0 = K0[2 * k], kn = KN[k];
+                const float2 k0 = K0[k], kn = KN[k];
                 const float2 ak = cmul(a, make_float2(k0.x, ksign * k0.y));
                 const float2 bk = cmul(b, make_float2(kn.x, ksign * kn.y));
                 v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
             }
         } else {
-            col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
+            col_kmul<PL>(v, K + (unsigned)(kx * H), t, ksign);
         }
         seq_sync<PL>();
     } else {
-        col_kmul<PL>(v, K + (unsigned)xpos(kx, 0, H), t, ksign);
+        col_kmul<PL>(v, K + (unsigned)(kx * H), t, ksign);
     }
 
     fft_regs<PL, +1>(v, row, t, g.twP[PL::variant]);
