@@ -557,6 +557,9 @@ constexpr int pow2_floor(int x)
 template <typename PL_> struct ColCfg {
     using PL = PL_;
     static_assert(plan_valid<PL>(), "invalid FFT plan");
+    // column lengths from the size rule (16 ... 8192 as listed in the column
+    // kernels) have complex-exchange plans only
+    static_assert(!PL::SPLIT, "column plans exchange complex values");
     static constexpr int H = PL::N;
     static constexpr int T = PL::T;
     static constexpr int P = PL::P;
@@ -588,55 +591,6 @@ __device__ __forceinline__ void col_kmul(float2* v, const float2* __restrict__ K
         const float2 k = Kr[slot_pos<PL, RL>(t, i)];
         v[i] = cmul(v[i], make_float2(k.x, ksign * k.y));
     }
-}
-
-// Split exchanges, the workgroup of kx = 0: sequence 0 holds the packed
-// DC/Nyquist spectrum C and forms A*K0 + i*B*KN from it with the same
-// operations as col_tile, through two float rows at once: its own row takes
-// the real parts, sequence 1's row (idle until that sequence's inverse FFT)
-// the imaginary parts.  The other sequences multiply by their kernel row.
-// The workgroup barriers order the borrowed row against sequence 1's own use
-// of it; grp is uniform over the workgroup and no sequence has left.
-template <typename PL>
-__device__ __forceinline__ void col_dc_nyquist_split(const ColGeom& g, float2* v, float* row,
-                                                     const float2* __restrict__ K, int seq, int t, int kx,
-                                                     float ksign)
-{
-    using C = ColCfg<PL>;
-    constexpr int H = C::H;
-    static_assert(C::SQ >= 2 && C::SQ <= 8, "split column plans hold 2 to 8 sequences per workgroup");
-    float* rim = row + C::L;
-    __syncthreads();
-    if (seq == 0) {
-#pragma unroll
-        for (int i = 0; i < C::P; ++i) {
-            const int k = lds_pad(slot_pos<PL, C::RL>(t, i));
-            row[k] = v[i].x;
-            rim[k] = v[i].y;
-        }
-    }
-    __syncthreads();
-    if (seq == 0) {
-        const float2* K0 = K;                         // kx = 0
-        const float2* KN = K + (unsigned)(g.N * H);  // kx = N, contiguous
-#pragma unroll
-        for (int i = 0; i < C::P; ++i) {
-            const int k = slot_pos<PL, C::RL>(t, i);
-            const int pk = lds_pad(k), pm = lds_pad(k == 0 ? 0 : H - k);
-            const float2 ck = make_float2(row[pk], rim[pk]);
-            const float2 cm = cconj(make_float2(row[pm], rim[pm]));
-            const float2 a = cscale(cadd(ck, cm), 0.5f);
-            const float2 d = csub(ck, cm);
-            const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
-            const float2 k0 = K0[k], kn = KN[k];
-            const float2 ak = cmul(a, make_float2(k0.x, ksign * k0.y));
-            const float2 bk = cmul(b, make_float2(kn.x, ksign * kn.y));
-            v[i] = make_float2(ak.x - bk.y, ak.y + bk.x);
-        }
-    } else {
-        col_kmul<PL>(v, K + (unsigned)(kx * H), t, ksign);
-    }
-    __syncthreads();
 }
 
 // Passes A / C on the kx-rows [grp*SQ, grp*SQ+SQ) of one plane:
@@ -672,10 +626,7 @@ __device__ __forceinline__ void col_tile(const ColGeom& g, const float2* Xin, fl
     for (int i = 0; i < C::P; ++i) v[i] = xin[2 * slot_pos<PL, C::RF>(t, i)];
     fft_regs<PL, -1>(v, row, t, g.twP[PL::variant]);
 
-    if constexpr (PL::SPLIT) {
-        if (grp == 0) col_dc_nyquist_split<PL>(g, v, row, K, seq, t, kx, ksign);
-        else col_kmul<PL>(v, K + (unsigned)(kx * H), t, ksign);
-    } else if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
+    if (grp == 0) {  // workgroup-uniform: this group holds the packed DC/Nyquist sequence
         if (seq == 0) {
 #pragma unroll
             for (int i = 0; i < C::P; ++i) row[lds_pad(slot_pos<PL, C::RL>(t, i))] = v[i];

@@ -45,7 +45,6 @@ CASES = [
     (1, 2, 50, 3200, 41, 41, 0, 16, 2),       # 128 x 3584 (row length 1792)
     (1, 2, 50, 1200, 41, 41, 0, 16, 2),       # 128 x 1536 (row length 768)
     (1, 2, 50, 2300, 41, 41, 0, 16, 2),       # 128 x 2560 (row length 1280)
-    (1, 2, 1200, 90, 41, 41, 0, 16, 2),       # 1280 x 256 (column length 1280: split exchanges)
     (2, 2, 100, 90, 16, 12, 0, 16, 5),        # 128 x 128, even kernel: explicit kernel-hat spectrum
     (2, 2, 90, 100, 15, 12, 0, 8, 4),         # 128 x 128, odd x even kernel: explicit, 8-bit
     (2, 3, 101, 77, 15, 15, 0, 16, 5),        # 128 x 128, odd data width (per-sample pad map loads)
@@ -93,7 +92,6 @@ VARIANT_CASES = [
     (1, 2, 1024, 1024, 41, 41),   # 1536 x 1536
     (1, 2, 2048, 300, 41, 41),    # 2560 x 512
     (1, 2, 90, 2300, 41, 41),     # 128 x 2560 (row length 1280)
-    (1, 2, 1200, 90, 41, 41),     # 1280 x 256 (column length 1280, split-exchange variants)
 ]
 
 
