@@ -32,8 +32,20 @@ __global__ __launch_bounds__(ColCfg<PL>::THREADS, PL::WPE) void col_conv_kernel(
     using C = ColCfg<PL>;
     __shared__ typename C::E lds[C::SQ * C::L];
     const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
-    // plane-fastest: consecutive logical blocks reuse the same kernel rows
-    const int p = lid % nplanes, grp = lid / nplanes;
+    int p, grp;
+    if constexpr (C::SQ == 1) {
+        // one column per workgroup: the two columns of a kx pair, which share
+        // every 16-byte word of the spectrum, run back to back so that their
+        // half-word writes meet in L2; then the planes, so that consecutive
+        // pairs of blocks reuse the same kernel columns (N is even)
+        const unsigned q = lid >> 1;
+        p = (int)(q % (unsigned)nplanes);
+        grp = (int)((q / (unsigned)nplanes) * 2u + (lid & 1u));
+    } else {
+        // plane-fastest: consecutive logical blocks reuse the same kernel rows
+        p = (int)(lid % (unsigned)nplanes);
+        grp = (int)(lid / (unsigned)nplanes);
+    }
     col_tile<PL>(g, Xin + p * xstride, Xout + p * xstride, K, grp, lds, ksign);
 }
 
@@ -75,6 +87,7 @@ static void launch_col_plan(const ColGeom& g, const float2* Xin, float2* Xout, l
 {
     using C = ColCfg<PL>;
     if (g.N % C::SQ && C::T > 64) throw_msg("row count not divisible by column group");
+    if (C::SQ == 1 && g.N % 2) throw_msg("single-column plans need an even column count");
     dim3 grid((unsigned)((g.N + C::SQ - 1) / C::SQ * nplanes));
     launch(col_conv_kernel<PL>, grid, dim3(C::THREADS), 0, s, g, Xin, Xout, xstride, K, nplanes, conj ? -1.0f : 1.0f);
 }
