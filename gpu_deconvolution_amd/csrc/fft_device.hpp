@@ -602,7 +602,8 @@ FMVD_HD void lds_out_slots(typename Comp<COMP>::E* lds, float2* v, int t)
 // twiddles), they are generated rather than loaded one by one: each thread
 // loads w^k and, for radices above 8, w^(Q k) with Q = ceil(sqrt(R)) from the
 // pass's table, and forms w^(k r) = w^(k a) w^(Q k m) (r = a + Q m) with the
-// powers built by binary splitting, so at most five rounded products deep.
+// powers built by binary splitting: for the radices up to 36 that the plans
+// use, at most four rounded products deep.
 // Butterflies whose k differ (k_b = k_0 + d_b, d_b a compile-time offset)
 // load their own pair of powers.  One or two loads per distinct k replace
 // NB (R - 1): the loads' data return, not the arithmetic, bounds the passes
